@@ -1,0 +1,84 @@
+// gd_rng.h -- counter-based random numbers shared by the host and gfx950 code.
+//
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with a Weyl key
+// schedule.  Stateless: a draw is a pure function of (key, counter), which is
+// what lets every workgroup re-derive a replica's move without shared state and
+// lets a job replay bit-identically alone or inside a batch.  It replaces the
+// reference's XORWOW states (backend_gpu/kernel_cuda_l1_initcurand.cu:4-10) and
+// the time-seeded rand_r of the CPU backend (kernel_cpp_montecarlo_soa.C:8-15).
+//
+// Stream layout used throughout the library:
+//   key     = (seed low 32 bits, seed high 32 bits)
+//   counter = (step, replica, job_id, slot)
+//   slot 0  -> move draws 0..3, slot 1 -> move draws 4..5, accept draw, spare
+//   slot 2  -> temperature-exchange draw (counter word 0 = round, word 1 = pair)
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define GD_HD __host__ __device__ __forceinline__
+#else
+#define GD_HD static inline
+#endif
+
+struct gd_u32x4 {
+  uint32_t v[4];
+};
+
+GD_HD void gd_mulhilo32(uint32_t a, uint32_t b, uint32_t &hi, uint32_t &lo) {
+  const uint64_t p = (uint64_t)a * (uint64_t)b;
+  hi = (uint32_t)(p >> 32);
+  lo = (uint32_t)p;
+}
+
+// Philox4x32 with R rounds; R = 10 is the standard generator.
+GD_HD gd_u32x4 gd_philox4x32(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                             uint32_t k1, int rounds = 10) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int r = 0; r < rounds; ++r) {
+    uint32_t hi0, lo0, hi1, lo1;
+    gd_mulhilo32(M0, c0, hi0, lo0);
+    gd_mulhilo32(M1, c2, hi1, lo1);
+    const uint32_t n0 = hi1 ^ c1 ^ k0;
+    const uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += W0;
+    k1 += W1;
+  }
+  gd_u32x4 out;
+  out.v[0] = c0;
+  out.v[1] = c1;
+  out.v[2] = c2;
+  out.v[3] = c3;
+  return out;
+}
+
+// 24-bit uniform in [0, 1): exact in FP32, identical on host and device.
+GD_HD float gd_u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// The eight draws of one replica-step: w[0..5] moves, w[6] accept, w[7] spare.
+GD_HD void gd_step_draws(uint64_t seed, uint32_t job, uint32_t replica, uint32_t step,
+                         uint32_t w[8]) {
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  const gd_u32x4 a = gd_philox4x32(step, replica, job, 0u, k0, k1);
+  const gd_u32x4 b = gd_philox4x32(step, replica, job, 1u, k0, k1);
+  w[0] = a.v[0];
+  w[1] = a.v[1];
+  w[2] = a.v[2];
+  w[3] = a.v[3];
+  w[4] = b.v[0];
+  w[5] = b.v[1];
+  w[6] = b.v[2];
+  w[7] = b.v[3];
+}
+
+// Exchange draw for pair `pair` in exchange round `round`.
+GD_HD float gd_exchange_u01(uint64_t seed, uint32_t job, uint32_t pair, uint32_t round) {
+  const gd_u32x4 a = gd_philox4x32(round, pair, job, 2u, (uint32_t)seed, (uint32_t)(seed >> 32));
+  return gd_u01(a.v[0]);
+}

@@ -1,0 +1,251 @@
+/*
+ * geauxdock_amd.h -- C ABI of the MI355X (gfx950) GeauxDock Monte Carlo hot path.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every entry point returns a
+ * gd_status; nothing calls exit().  The library owns device memory; callers own
+ * every host array they pass in.
+ *
+ * Reference interfaces replaced (file:line in geaux-computational-bio/geauxdock):
+ *   gd_job_create        <- CopyH2D / DeviceAlloc      src/backend_gpu/prepair.h:5,8
+ *   gd_dock              <- Dock (GPU backend)          src/backend_gpu/backend_gpu.h:18-23
+ *   gd_energy_eval       <- the s2max==1 initial-energy launch of MonteCarlo_d
+ *                           src/backend_gpu/backend_gpu.cu:87-91 (kernel_cuda_l1_montecarlo.cu:29)
+ *   gd_rng_seed          <- InitCurand                  src/backend_gpu/backend_gpu.h:16
+ *   gd_set_temperature   <- SetTemperature              src/frontend/util.C:325-363
+ *   gd_load_complex      <- server1.C job assembly      src/frontend/server1.C:54-217
+ *   gd_batch_dock        <- (no counterpart: many complexes in one launch sequence)
+ *   gd_server_*          <- (no counterpart: continuous-batching request loop;
+ *                           supersedes the MPI job farm src/main_mpi/server.C:27-66)
+ * The reference's own C++ signatures (SetDevice, DeviceAlloc, ... Dock) are
+ * exported too, see geauxdock_dropin.h.
+ */
+#ifndef GEAUXDOCK_AMD_H
+#define GEAUXDOCK_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Table dimensions of the force field (reference size.h:64-85). */
+#define GD_NPT 31        /* protein effective-point types (MAXTP1) */
+#define GD_NLT 24        /* ligand SYBYL atom types (MAXTP2) */
+#define GD_NRES 20       /* residue types (MAXTP4) */
+#define GD_NELE 50       /* electrostatic table size (MAXTP3) */
+#define GD_NTERM 10      /* energy terms incl. total (MAXWEI) */
+#define GD_MCS_INVALID 0.01234f  /* unmatched MCS position sentinel (size.h:107) */
+#define GD_BADTYPE 50    /* unknown SYBYL type code (size.h:101) */
+
+/* Energy term indices (reference geauxdock.h:131-147). */
+enum { GD_VDW = 0, GD_ELE, GD_PMF, GD_PSP, GD_HDB, GD_HPC, GD_KDE, GD_LHM, GD_DST, GD_TOTAL };
+
+typedef enum {
+  GD_OK = 0,
+  GD_ERR_ARG = 1,      /* NULL pointer, negative size, inconsistent arguments */
+  GD_ERR_HIP = 2,      /* a HIP runtime call failed (gd_last_error() has the text) */
+  GD_ERR_SIZE = 3,     /* size beyond a documented limit */
+  GD_ERR_TYPE = 4,     /* unknown atom / point / residue type (e.g. BADKDE) */
+  GD_ERR_IO = 5,       /* file missing or unparsable */
+  GD_ERR_NOMEM = 6,    /* device or host allocation failed */
+  GD_ERR_STATE = 7,    /* call out of order (e.g. no device initialised) */
+  GD_ERR_BUSY = 8,     /* serving: admission refused (HBM budget) */
+  GD_ERR_NODEV = 9     /* no usable gfx950 device */
+} gd_status;
+
+/* Derived force field, reference EnePara semantics (geauxdock_soa.h:154-187,
+ * produced by OptimizeEnepara util_optimize_soa.C:489-540).  Tables are
+ * indexed [ligand type][protein point type]. */
+typedef struct {
+  float p1a[GD_NLT][GD_NPT];
+  float p2a[GD_NLT][GD_NPT];
+  float lj0, lj1;
+  float el1, el0, a1, b1;
+  float pmf0[GD_NLT][GD_NPT];
+  float pmf1[GD_NLT][GD_NPT];
+  float hdb0[GD_NLT][GD_NPT];
+  float hdb1[GD_NLT][GD_NPT];
+  float hpp[GD_NRES];
+  float hpl0[GD_NLT], hpl1[GD_NLT], hpl2[GD_NLT];
+  float kde2, kde3;
+  float a_para[GD_NTERM], b_para[GD_NTERM], w[GD_NTERM];
+  float ele[GD_NELE];   /* raw electrostatic table, used by the protein loader */
+} gd_enepara;
+
+/* One docking complex ("job"): all arrays are host memory owned by the caller
+ * and only read during gd_job_create.  Atom order is the reference's
+ * type-sorted order (x3/t3/c3 of geauxdock_soa.h:30-34); conformers share the
+ * atom types/charges, protein conformers share every point attribute. */
+typedef struct {
+  int n_lig, n_prt, n_tmp;                 /* ensemble sizes; n_rep = n_lig*n_prt*n_tmp */
+  int lig_natom, prt_npoint, kde_npoint, mcs_nrow;
+  /* ligand conformers [n_lig][lig_natom], relative to the conformer centroid */
+  const float *lig_x, *lig_y, *lig_z;
+  const int *lig_t;                        /* [lig_natom] SYBYL code 0..23 */
+  const float *lig_c;                      /* [lig_natom] partial charge */
+  const int *lig_perm;                     /* [lig_natom] file-order index of each atom, or NULL */
+  float lig_center[3];                     /* native centroid in the lab frame */
+  /* protein: coordinates [n_prt][prt_npoint]; attributes [prt_npoint] */
+  const float *prt_x, *prt_y, *prt_z;
+  const int *prt_t, *prt_c, *prt_cdc, *prt_seq3r;
+  const float *prt_hpp, *prt_ele;
+  float pocket_center[3];
+  /* pocket-specific potential psp[ligand type][residue serial], [GD_NLT][psp_ncol] */
+  const float *psp;
+  int psp_ncol;
+  /* KDE anchor points, sorted by type [kde_npoint] */
+  const int *kde_t;
+  const float *kde_x, *kde_y, *kde_z;
+  /* MCS restraints [mcs_nrow][lig_natom] (same atom order as the ligand),
+   * x == GD_MCS_INVALID marks an unmatched atom; tcc weight [mcs_nrow] */
+  const float *mcs_x, *mcs_y, *mcs_z, *mcs_tcc;
+  const gd_enepara *ene;
+  const float *minus_beta;                 /* [n_tmp] = -1/T */
+} gd_job_desc;
+
+enum { GD_MOVE_REF_UNIFORM = 0,   /* reference: m += scale * U[0,1) (kernel_cpp_montecarlo_soa.C:248-249) */
+       GD_MOVE_GAUSSIAN = 1 };    /* m += scale * N(0,1) */
+
+typedef struct {
+  int steps_total;          /* MC steps after the initial-energy pass */
+  int steps_per_dump;       /* steps per record segment (reference STEPS_PER_DUMP) */
+  int steps_per_exchange;   /* temperature exchange period; 0 = off */
+  int move_mode;            /* GD_MOVE_* */
+  float move_scale[6];      /* tx ty tz rx ry rz */
+  uint64_t seed;            /* Philox4x32-10 key */
+  uint32_t job_id;          /* Philox counter word 2: the job's stream */
+  int record;               /* 1 = keep per-dump accept records (drop-in Dock) */
+} gd_mc_param;
+
+/* Per-replica results (any pointer may be NULL).  Replica order is the
+ * reference SetReplica order idx = n_tmp*n_lig*p + n_lig*t + l (util.C:380). */
+typedef struct {
+  float *movematrix;        /* [n_rep][6] last accepted pose */
+  float *energy;            /* [n_rep][10] last accepted terms (normalised) + total */
+  int *idx_tmp;             /* [n_rep] temperature index after exchanges */
+  int *n_accept;            /* [n_rep] accepted MC moves */
+  int *n_exchange;          /* [n_rep] accepted temperature swaps */
+  float *best_energy;       /* [n_rep] lowest accepted total energy */
+  int *is_accept;           /* [n_rep] last step accepted */
+} gd_replica_out;
+
+typedef struct {
+  double ms_total;          /* wall time of the MC loop (host clock) */
+  double ms_pair_kernel;    /* summed HIP-event time of the pair-energy kernel */
+  double ms_finish_kernel;  /* summed HIP-event time of the finish kernel */
+  int n_pair_launch;        /* pair-kernel launches timed */
+  double pair_evals;        /* ligand-atom x protein-point evaluations done */
+  double replica_steps;     /* replica x MC steps done (initial pass excluded) */
+} gd_stats;
+
+typedef struct gd_job gd_job;         /* device-resident complex */
+typedef struct gd_server gd_server;   /* serving loop */
+
+/* ---- device ---------------------------------------------------------- */
+gd_status gd_init(int device);                 /* select + check a gfx950 device */
+gd_status gd_device_info(char *name, int name_len, int *n_cu, int *clock_khz,
+                         size_t *hbm_bytes);
+const char *gd_last_error(void);
+const char *gd_version(void);
+
+/* ---- host helpers (reference frontend semantics) ---------------------- */
+/* Geometric ladder T_i = floor * r^i (util.C:325-363); n == 1 uses floor. */
+gd_status gd_set_temperature(int n_tmp, float floor_temp, float ceiling_temp,
+                             float *minus_beta_out);
+
+/* ---- jobs ------------------------------------------------------------- */
+gd_status gd_job_create(const gd_job_desc *desc, gd_job **out);
+gd_status gd_job_destroy(gd_job *job);
+gd_status gd_job_sizes(const gd_job *job, int *n_rep, int *lig_natom, int *prt_npoint,
+                       int *kde_npoint, int *mcs_nrow);
+size_t gd_job_device_bytes(const gd_job *job);
+
+/* Energy of arbitrary poses.  conf[n][2] = (ligand conformer, protein
+ * conformer); pose[n][6] = movematrix.  terms[n][10] are the normalised
+ * terms + total (what ReplicaMC.energy holds); raw[n][9] (optional) the
+ * pre-normalisation sums vdw, ele, pmf, psp, -hdb/sqrt(2pi), hpc, kde/kde3,
+ * lhm, dst; coords[n][lig_natom][3] (optional) the transformed ligand. */
+gd_status gd_energy_eval(gd_job *job, int n_pose, const int *conf, const float *pose,
+                         float *terms, float *raw, float *coords);
+
+/* Plain one-job docking: initial-energy pass, then steps_total MC steps with
+ * temperature exchange, replica state resident on the device. */
+gd_status gd_dock(gd_job *job, const gd_mc_param *mc, gd_replica_out *out, gd_stats *stats);
+
+/* Accept records of the last dump segment: rec[n_rep][cap] entries of
+ * (step, movematrix[6], energy[10]) packed as 17 floats; count[n_rep]. */
+gd_status gd_dock_records(gd_job *job, int cap, float *rec, int *count);
+
+/* Many complexes in one launch sequence (persistent work-queue pair kernel,
+ * graph-captured sweep segments).  mc[i] / out[i] per job. */
+gd_status gd_batch_dock(gd_job **jobs, int n_jobs, const gd_mc_param *mc,
+                        gd_replica_out *out, gd_stats *stats);
+
+/* ---- tiling / autotuner ------------------------------------------------ */
+typedef struct {
+  int atoms_per_lane;       /* ligand atoms per work item (4, 8 or 16) */
+  int tile_points;          /* protein points per tile */
+  int waves_per_cu;         /* persistent grid size per CU */
+} gd_tiling;
+gd_status gd_tiling_get(const gd_job *job, gd_tiling *t);
+gd_status gd_tiling_set(gd_job *job, const gd_tiling *t);
+/* Sweep the tiling grid on `job` for `steps` MC steps each and keep the
+ * fastest; results cached by (protein-size, ligand-size) bucket. */
+gd_status gd_autotune(gd_job *job, int steps, gd_tiling *best, double *best_ms);
+
+/* ---- serving ------------------------------------------------------------ */
+typedef struct {
+  size_t hbm_budget;        /* bytes of device memory jobs may occupy */
+  int max_active;           /* cap on concurrently active jobs */
+  int sweep_steps;          /* MC steps per sweep between admissions */
+} gd_server_config;
+
+typedef struct {
+  int64_t ticket;
+  int status;               /* gd_status of the job */
+  double t_submit, t_admit, t_done;   /* seconds, server clock */
+  double replica_steps;
+} gd_job_result;
+
+gd_status gd_server_create(const gd_server_config *cfg, gd_server **out);
+gd_status gd_server_destroy(gd_server *s);
+/* Takes ownership of `job`; out (optional) receives the final replica state
+ * when the job retires (arrays must stay valid until then). */
+gd_status gd_server_submit(gd_server *s, gd_job *job, const gd_mc_param *mc,
+                           gd_replica_out *out, int64_t *ticket);
+/* One sweep: admit what fits, advance every active job, retire finished
+ * ones.  *n_done receives how many results are now ready. */
+gd_status gd_server_step(gd_server *s, int *n_active, int *n_done);
+gd_status gd_server_cancel(gd_server *s, int64_t ticket);
+/* Pop up to `cap` finished results. */
+gd_status gd_server_results(gd_server *s, gd_job_result *res, int cap, int *n);
+double gd_server_clock(gd_server *s);
+
+/* ---- loaders (reference frontend, re-implemented) ----------------------- */
+typedef struct gd_host_job gd_host_job;   /* host-side complex built by the loaders */
+/* Loads paras + 08ff_opt + 08_nor_a/b (loadEnePara/loadWeight/loadNorPara +
+ * OptimizeEnepara, load.C:1189-1507, util_optimize_soa.C:489-540). */
+gd_status gd_load_enepara(const char *paras, const char *weight, const char *nor_a,
+                          const char *nor_b, gd_enepara *out);
+/* One ligand .sdf + its eFindSite .ff + protein .pdb, as server1.C builds a
+ * Complex: loadLigand/loadLHM/loadProtein + Optimize*.  n_tmp temperatures
+ * on the given ladder. */
+gd_status gd_load_complex(const char *sdf, const char *ff, const char *pdb,
+                          const char *ligand_id, const gd_enepara *ene, int n_tmp,
+                          float floor_temp, float ceiling_temp, gd_host_job **out);
+/* Synthetic complex (seeded, deterministic): prt_npoint effective points,
+ * lig_natom heavy atoms, ensemble sizes as given. */
+gd_status gd_synth_complex(int prt_npoint, int lig_natom, int n_lig, int n_prt, int n_tmp,
+                           float floor_temp, float ceiling_temp, uint64_t seed,
+                           const gd_enepara *ene, gd_host_job **out);
+/* View of a host job as a gd_job_desc (pointers stay valid until free). */
+gd_status gd_host_job_desc(const gd_host_job *h, gd_job_desc *desc);
+gd_status gd_host_job_free(gd_host_job *h);
+/* Conformer RMSD after optimal superposition (replaces rmsd.f u3b, load.C:421). */
+float gd_kabsch_rmsd(int n, const float *a_xyz, const float *b_xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,135 @@
+// ref_harness.cpp -- TEST INFRASTRUCTURE ONLY (see oracle/gd_oracle.cpp).
+//
+// Drives the reference CPU backend, compiled unmodified from
+// /root/reference/src/backend_cpu_mic/backend_cpu_mic.C (which includes
+// kernel_cpp_montecarlo_soa.C and the energy_*_v1.C fragments), on a job
+// described by a gd_job_desc: it fills the reference's own Complex struct
+// (src/common/geauxdock.h:391-439, the layout comes from the reference
+// headers on the include path), stores each requested pose in a replica's
+// movematrix and calls the reference Dock with steps_total = 0, which runs
+// only the RNG-free initial-energy pass (backend_cpu_mic.C:53-56): the
+// move amount is 0 and the accept draw is 0, so the energy of the stored pose
+// lands in replica[r].energy.  A temperature of -minus_beta = 1e-30 keeps the
+// forced accept from underflowing.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include <geauxdock.h>
+#include <backend_cpu_mic.h>
+
+#include "../../include/geauxdock_amd.h"
+
+extern "C" int ref_energy(const gd_job_desc *d, int n, const int *conf, const float *pose,
+                          float *terms) {
+  const int L = d->lig_natom, P = d->prt_npoint, K = d->kde_npoint, M = d->mcs_nrow;
+  if (L > MAXLIG || P > MAXPRO || K > MAXKDE || M > MAX_MCS_ROW || d->n_lig > MAX_CONF_LIG ||
+      d->n_prt > MAX_CONF_PRT || n > MAX_REP || d->psp_ncol > MAXPRO)
+    return 3;  // outside the reference's fixed capacities
+  // the reference types are 64-byte aligned (geauxdock.h:29); -O3 vector
+  // code relies on it, so allocate accordingly and zero-fill
+  const size_t cb = (sizeof(Complex) + 63) / 64 * 64, rb = ((size_t)n * sizeof(Record) + 63) / 64 * 64;
+  Complex *c = (Complex *)aligned_alloc(64, cb);
+  Record *rec = (Record *)aligned_alloc(64, rb);
+  if (!c || !rec) return 6;
+  memset((void *)c, 0, cb);
+  memset((void *)rec, 0, rb);
+  for (int i = 0; i < d->n_lig; ++i) {
+    Ligand *g = &c->lig[i];
+    g->lig_natom = L;
+    for (int k = 0; k < 3; ++k) g->center[k] = d->lig_center[k];
+    for (int j = 0; j < L; ++j) {
+      const float x = d->lig_x[(size_t)i * L + j], y = d->lig_y[(size_t)i * L + j],
+                  z = d->lig_z[(size_t)i * L + j];
+      const int f = d->lig_perm ? d->lig_perm[j] : j;
+      g->x1[j] = g->x3[j] = x;
+      g->y1[j] = g->y3[j] = y;
+      g->z1[j] = g->z3[j] = z;
+      g->t1[j] = g->t3[j] = d->lig_t[j];
+      g->c1[j] = g->c3[j] = d->lig_c[j];
+      g->x2[f] = x, g->y2[f] = y, g->z2[f] = z;
+      g->t2[f] = d->lig_t[j], g->c2[f] = d->lig_c[j];
+    }
+  }
+  for (int i = 0; i < d->n_prt; ++i) {
+    Protein *p = &c->prt[i];
+    p->prt_npoint = P;
+    for (int k = 0; k < 3; ++k) p->pocket_center[k] = d->pocket_center[k];
+    for (int j = 0; j < P; ++j) {
+      p->t[j] = d->prt_t[j];
+      p->x[j] = d->prt_x[(size_t)i * P + j];
+      p->y[j] = d->prt_y[(size_t)i * P + j];
+      p->z[j] = d->prt_z[(size_t)i * P + j];
+      p->cdc[j] = d->prt_cdc[j];
+      p->hpp[j] = d->prt_hpp[j];
+      p->ele[j] = d->prt_ele[j];
+      p->c[j] = d->prt_c[j];
+      p->seq3r[j] = d->prt_seq3r[j];
+    }
+  }
+  for (int lt = 0; lt < GD_NLT && lt < MAXLIG; ++lt)
+    for (int s = 0; s < d->psp_ncol; ++s) c->psp.psp[lt][s] = d->psp[(size_t)lt * d->psp_ncol + s];
+  c->kde.kde_npoint = K;
+  for (int k = 0; k < K; ++k) {
+    c->kde.t[k] = d->kde_t[k];
+    c->kde.x[k] = d->kde_x[k];
+    c->kde.y[k] = d->kde_y[k];
+    c->kde.z[k] = d->kde_z[k];
+  }
+  for (int m = 0; m < M; ++m) {
+    Mcs *r = &c->mcs[m];
+    int nc = 0;
+    for (int j = 0; j < MAX_MCS_COL; ++j) r->x[j] = r->y[j] = r->z[j] = MCS_INVALID_COORD;
+    for (int j = 0; j < L; ++j) {
+      const int f = d->lig_perm ? d->lig_perm[j] : j;
+      r->x[f] = d->mcs_x[(size_t)m * L + j];
+      r->y[f] = d->mcs_y[(size_t)m * L + j];
+      r->z[f] = d->mcs_z[(size_t)m * L + j];
+      nc += d->mcs_x[(size_t)m * L + j] != MCS_INVALID_COORD;
+    }
+    r->ncol = nc;
+    r->tcc = d->mcs_tcc[m];
+  }
+  const gd_enepara *e = d->ene;
+  EnePara *ep = &c->enepara;
+  memcpy(ep->p1a, e->p1a, sizeof ep->p1a);
+  memcpy(ep->p2a, e->p2a, sizeof ep->p2a);
+  memcpy(ep->pmf0, e->pmf0, sizeof ep->pmf0);
+  memcpy(ep->pmf1, e->pmf1, sizeof ep->pmf1);
+  memcpy(ep->hdb0, e->hdb0, sizeof ep->hdb0);
+  memcpy(ep->hdb1, e->hdb1, sizeof ep->hdb1);
+  ep->lj0 = e->lj0, ep->lj1 = e->lj1, ep->el1 = e->el1, ep->el0 = e->el0;
+  ep->a1 = e->a1, ep->b1 = e->b1;
+  memcpy(ep->hpp, e->hpp, sizeof ep->hpp);
+  memcpy(ep->hpl0, e->hpl0, sizeof ep->hpl0);
+  memcpy(ep->hpl1, e->hpl1, sizeof ep->hpl1);
+  memcpy(ep->hpl2, e->hpl2, sizeof ep->hpl2);
+  ep->kde2 = e->kde2, ep->kde3 = e->kde3;
+  memcpy(ep->a_para, e->a_para, sizeof ep->a_para);
+  memcpy(ep->b_para, e->b_para, sizeof ep->b_para);
+  memcpy(ep->w, e->w, sizeof ep->w);
+  c->temp[0].minus_beta = -1e-30f;
+  for (int r = 0; r < n; ++r) {
+    ReplicaMC *rep = &c->replica[r];
+    rep->idx_rep = r;
+    rep->idx_lig = conf[2 * r];
+    rep->idx_prt = conf[2 * r + 1];
+    rep->idx_tmp = 0;
+    for (int k = 0; k < 6; ++k) rep->movematrix[k] = pose[6 * r + k];
+  }
+  c->size.n_lig = d->n_lig, c->size.n_prt = d->n_prt, c->size.n_tmp = 1, c->size.n_rep = n;
+  c->size.lig_natom = L, c->size.prt_npoint = P, c->size.kde_npoint = K, c->size.mcs_nrow = M;
+  c->mcpara.steps_total = 0;
+  c->mcpara.steps_per_dump = 1;
+  c->rep_begin = 0;
+  c->rep_end = n - 1;
+  std::strcpy(c->lig_file, "job");
+  std::strcpy(c->prt_file, "job");
+  Dock(c, rec);
+  for (int r = 0; r < n; ++r)
+    for (int k = 0; k < MAXWEI; ++k) terms[(size_t)r * MAXWEI + k] = c->replica[r].energy[k];
+  free(rec);
+  free(c);
+  return 0;
+}
