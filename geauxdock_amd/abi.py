@@ -115,8 +115,10 @@ PROTOTYPES = {
     "gd_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(gd_tiling), C.POINTER(C.c_double)]),
     "gd_server_create": (C.c_int, [C.POINTER(gd_server_config), C.POINTER(_vp)]),
     "gd_server_destroy": (C.c_int, [_vp]),
-    "gd_server_submit": (C.c_int, [_vp, _vp, C.POINTER(gd_mc_param), C.POINTER(gd_replica_out),
-                                   C.POINTER(C.c_int64)]),
+    "gd_server_submit": (C.c_int, [_vp, C.POINTER(gd_job_desc), C.POINTER(gd_mc_param),
+                                   C.POINTER(gd_replica_out), C.POINTER(C.c_int64)]),
+    "gd_job_footprint": (C.c_size_t, [C.POINTER(gd_job_desc)]),
+    "gd_server_usage": (C.c_int, [_vp, C.POINTER(C.c_size_t), _ip]),
     "gd_server_step": (C.c_int, [_vp, _ip, _ip]),
     "gd_server_cancel": (C.c_int, [_vp, C.c_int64]),
     "gd_server_results": (C.c_int, [_vp, C.POINTER(gd_job_result), C.c_int, _ip]),

@@ -1,0 +1,641 @@
+// gd_batch.hip -- many complexes in shared launches: batched docking
+// (persistent work-queue pair kernel, graph-captured sweep segments), the
+// tiling autotuner, and the continuous-batching serving loop.
+//
+// A "batch" concatenates the device structs of its jobs into one array, and
+// their work items and 64-replica blocks into one list each, so one pair
+// launch and one finish launch advance every job by one MC step.  Each job
+// keeps its own tiling, replica order, Philox stream (seed, job_id) and step
+// counter (launch step - step0), so a job computes bit-identical results
+// alone or inside any batch.  The reference has no counterpart: its MPI
+// farm (src/main_mpi/server.C:27-66, client_gpu.C:42-72) runs one complex per
+// GPU launch sequence.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "gd_internal.h"
+
+template <int LC>
+__global__ void gd_pair_kernel_wq(const GdJob *, const GdItem *, int, int *);
+__global__ void gd_transform_kernel(const GdJob *, const GdBlock *, int *);
+__global__ void gd_finish_kernel(const GdJob *, const GdBlock *, int, int, const int *, int *);
+__global__ void gd_exchange_kernel(const GdJob *, const GdExch *);
+
+using namespace gdr;
+
+namespace {
+
+hipError_t launch_pair_wq(int lc, const GdJob *jobs, const GdItem *items, int n_items, int *wq,
+                          int grid, hipStream_t s) {
+  if (n_items <= 0) return hipSuccess;
+  const dim3 g(grid), b(256);
+  switch (lc) {
+    case 4:
+      hipLaunchKernelGGL(gd_pair_kernel_wq<4>, g, b, 0, s, jobs, items, n_items, wq);
+      break;
+    case 8:
+      hipLaunchKernelGGL(gd_pair_kernel_wq<8>, g, b, 0, s, jobs, items, n_items, wq);
+      break;
+    default:
+      hipLaunchKernelGGL(gd_pair_kernel_wq<16>, g, b, 0, s, jobs, items, n_items, wq);
+      break;
+  }
+  return hipGetLastError();
+}
+
+// Device-side lists of one batch: job structs, items, blocks, exchange table.
+struct BatchLists {
+  GdJob *d_jobs = nullptr;
+  GdItem *d_items = nullptr;
+  GdBlock *d_blocks = nullptr;
+  int *d_wq = nullptr;       // work-queue counter
+  int *d_base = nullptr;     // base launch step for graph replays
+  GdExch *d_ex = nullptr;    // exchange entries (scratch, per launch)
+  int n_items = 0, n_blocks = 0, n_jobs = 0, ex_cap = 0;
+  void *mem = nullptr;
+  ~BatchLists() {
+    if (mem) (void)hipFree(mem);
+  }
+  gd_status build(const std::vector<gd_job *> &jobs, const std::vector<int> &which,
+                  hipStream_t s) {
+    std::vector<GdJob> js;
+    std::vector<GdItem> items;
+    std::vector<GdBlock> blocks;
+    for (int k = 0; k < (int)which.size(); ++k) {
+      gd_job *j = jobs[which[k]];
+      js.push_back(j->h);
+      // job-local items in protein-tile-major order: waves that start together
+      // share the same protein tile (scalar-cache and L2 reuse)
+      for (const GdItem &it : j->items) items.push_back({k, it.rb, it.a, it.b});
+      for (int rb = 0; rb < j->h.Rp; rb += 64) blocks.push_back({k, rb});
+    }
+    // longest items first: the persistent queue then ends with small items
+    std::stable_sort(items.begin(), items.end(), [&](const GdItem &a, const GdItem &b) {
+      const GdJob &ja = js[a.job], &jb = js[b.job];
+      return (long long)ja.ptile * ja.lc > (long long)jb.ptile * jb.lc;
+    });
+    n_jobs = (int)js.size();
+    n_items = (int)items.size();
+    n_blocks = (int)blocks.size();
+    ex_cap = std::max(1, n_jobs);
+    const size_t bj = (js.size() * sizeof(GdJob) + 255) / 256 * 256;
+    const size_t bi = (items.size() * sizeof(GdItem) + 255) / 256 * 256;
+    const size_t bb = (blocks.size() * sizeof(GdBlock) + 255) / 256 * 256;
+    const size_t be = ((size_t)ex_cap * sizeof(GdExch) + 255) / 256 * 256;
+    if (mem) (void)hipFree(mem);
+    mem = nullptr;
+    hipError_t e = hipMalloc(&mem, bj + bi + bb + be + 512);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc batch");
+    char *p = (char *)mem;
+    d_jobs = (GdJob *)p;
+    d_items = (GdItem *)(p + bj);
+    d_blocks = (GdBlock *)(p + bj + bi);
+    d_ex = (GdExch *)(p + bj + bi + bb);
+    d_wq = (int *)(p + bj + bi + bb + be);
+    d_base = d_wq + 64;
+    if ((e = hipMemcpyAsync(d_jobs, js.data(), js.size() * sizeof(GdJob), hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(GdItem), hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(d_blocks, blocks.data(), blocks.size() * sizeof(GdBlock), hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemsetAsync(d_wq, 0, 512, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+      return hip_fail(e, "batch upload");
+    return GD_OK;
+  }
+};
+
+int pair_grid(int waves_per_cu) {
+  const int cu = std::max(1, n_cu());
+  return std::max(1, cu * std::max(4, waves_per_cu) / 4);
+}
+
+// One MC step of every job in the lists (pair + finish), optional exchange.
+gd_status batch_step(const BatchLists &B, int lc, int grid, int launch_step, const int *base,
+                     const std::vector<GdExch> *ex, hipStream_t s, KTimer *tp) {
+  if (tp && tp->begin(s) != GD_OK) return GD_ERR_HIP;
+  hipError_t e = launch_pair_wq(lc, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s);
+  if (e != hipSuccess) return hip_fail(e, "pair kernel (queue) launch");
+  if (tp && tp->end(s) != GD_OK) return GD_ERR_HIP;
+  hipLaunchKernelGGL(gd_finish_kernel, dim3(B.n_blocks), dim3(256), 0, s, B.d_jobs, B.d_blocks,
+                     GD_MODE_STEP, launch_step, base, B.d_wq);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "finish kernel launch");
+  if (ex && !ex->empty()) {
+    e = hipMemcpyAsync(B.d_ex, ex->data(), ex->size() * sizeof(GdExch), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync exch");
+    hipLaunchKernelGGL(gd_exchange_kernel, dim3((int)ex->size()), dim3(256), 0, s, B.d_jobs, B.d_ex);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "exchange kernel launch");
+  }
+  return GD_OK;
+}
+
+gd_status batch_init(const BatchLists &B, int lc, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(gd_transform_kernel, dim3(B.n_blocks), dim3(256), 0, s, B.d_jobs, B.d_blocks,
+                     B.d_wq);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "transform kernel launch");
+  if ((e = launch_pair_wq(lc, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s)) != hipSuccess)
+    return hip_fail(e, "pair kernel (queue) launch");
+  hipLaunchKernelGGL(gd_finish_kernel, dim3(B.n_blocks), dim3(256), 0, s, B.d_jobs, B.d_blocks,
+                     GD_MODE_INIT, 0, (const int *)nullptr, B.d_wq);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "finish kernel launch");
+  return GD_OK;
+}
+
+// Bring every job to one ligand-chunk width (one kernel template per launch).
+gd_status common_lc(const std::vector<gd_job *> &jobs, int &lc) {
+  lc = jobs.empty() ? 8 : jobs[0]->h.lc;
+  bool same = true;
+  for (gd_job *j : jobs) same = same && j->h.lc == lc;
+  if (same) return GD_OK;
+  lc = 8;
+  for (gd_job *j : jobs)
+    if (j->h.lc != lc) {
+      gd_status st = apply_tiling(j, lc, j->h.ptile);
+      if (st != GD_OK) return st;
+      if ((st = sync_job_struct(j)) != GD_OK) return st;
+    }
+  return GD_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" gd_status gd_batch_dock(gd_job **jobs_in, int n_jobs, const gd_mc_param *mc,
+                                   gd_replica_out *out, gd_stats *stats) {
+  if (!jobs_in || n_jobs <= 0 || !mc) return GD_ERR_ARG;
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  std::vector<gd_job *> jobs(jobs_in, jobs_in + n_jobs);
+  for (gd_job *j : jobs)
+    if (!j) return GD_ERR_ARG;
+  int lc;
+  if ((st = common_lc(jobs, lc)) != GD_OK) return st;
+  hipStream_t s = jobs[0]->stream;
+  int steps_max = 0;
+  for (int k = 0; k < n_jobs; ++k) {
+    if (mc[k].steps_total < 0 || mc[k].steps_per_exchange < 0) return GD_ERR_ARG;
+    set_mc(jobs[k]->h, &mc[k]);
+    jobs[k]->h.rec = nullptr;
+    jobs[k]->h.rec_cap = 0;
+    jobs[k]->h.record = 0;
+    if ((st = reset_replicas(jobs[k])) != GD_OK) return st;
+    steps_max = std::max(steps_max, mc[k].steps_total);
+  }
+  // jobs run from launch step 0; a job whose steps are done leaves the lists
+  std::vector<int> active(n_jobs);
+  for (int k = 0; k < n_jobs; ++k) active[k] = k;
+  BatchLists B;
+  if ((st = B.build(jobs, active, s)) != GD_OK) return st;
+  const int grid = pair_grid(16);
+  KTimer tp;
+  const auto t0 = std::chrono::steady_clock::now();
+  if ((st = batch_init(B, lc, grid, s)) != GD_OK) return st;
+
+  // Segments of identical shape are captured once into a HIP graph and
+  // replayed with the base step in device memory (no memset nodes).
+  int seg = 0;
+  bool uniform = true;
+  for (int k = 0; k < n_jobs; ++k)
+    uniform = uniform && mc[k].steps_total == mc[0].steps_total &&
+              mc[k].steps_per_exchange == mc[0].steps_per_exchange;
+  if (uniform) seg = mc[0].steps_per_exchange > 0 ? mc[0].steps_per_exchange : 50;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  double pair_evals = 0.0, rep_steps = 0.0;
+  for (int k = 0; k < n_jobs; ++k) {
+    const GdJob &g = jobs[k]->h;
+    pair_evals += (double)g.R * g.L * g.P * (mc[k].steps_total + 1);
+    rep_steps += (double)g.R * mc[k].steps_total;
+  }
+  int step = 0;
+  std::vector<GdExch> ex;
+  while (step < steps_max && st == GD_OK) {
+    // graph path: whole segment, every job active, exchange at its end
+    if (uniform && seg > 1 && step + seg <= steps_max && !stats) {
+      hipError_t e = hipMemcpyAsync(B.d_base, &step, sizeof(int), hipMemcpyHostToDevice, s);
+      if (e != hipSuccess) {
+        st = hip_fail(e, "hipMemcpyAsync base");
+        break;
+      }
+      if (!gexec) {
+        if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+          st = hip_fail(e, "hipStreamBeginCapture");
+          break;
+        }
+        for (int k = 0; k < seg && st == GD_OK; ++k) {
+          std::vector<GdExch> exk;
+          if (k == seg - 1 && mc[0].steps_per_exchange > 0)
+            for (int q = 0; q < n_jobs; ++q)
+              if (jobs[q]->h.n_tmp > 1) exk.push_back({q, -1});
+          // exchange entries with round -1 take the round from the base step
+          st = batch_step(B, lc, grid, k, B.d_base, nullptr, s, nullptr);
+          if (st == GD_OK && !exk.empty()) {
+            // round of this segment = (base + seg) / seg - 1, resolved at replay
+            // by re-uploading the table before the graph launch (below)
+            hipLaunchKernelGGL(gd_exchange_kernel, dim3((int)exk.size()), dim3(256), 0, s,
+                               B.d_jobs, B.d_ex);
+          }
+        }
+        hipError_t e2 = hipStreamEndCapture(s, &graph);
+        if (st == GD_OK && e2 != hipSuccess) st = hip_fail(e2, "hipStreamEndCapture");
+        if (st != GD_OK) break;
+        if ((e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0)) != hipSuccess) {
+          st = hip_fail(e, "hipGraphInstantiate");
+          break;
+        }
+      }
+      if (mc[0].steps_per_exchange > 0) {
+        ex.clear();
+        const int round = (step + seg) / seg - 1;
+        for (int q = 0; q < n_jobs; ++q)
+          if (jobs[q]->h.n_tmp > 1) ex.push_back({q, round});
+        if (!ex.empty() &&
+            (e = hipMemcpyAsync(B.d_ex, ex.data(), ex.size() * sizeof(GdExch), hipMemcpyHostToDevice, s)) != hipSuccess) {
+          st = hip_fail(e, "hipMemcpyAsync exch");
+          break;
+        }
+      }
+      if ((e = hipGraphLaunch(gexec, s)) != hipSuccess) {
+        st = hip_fail(e, "hipGraphLaunch");
+        break;
+      }
+      // the host-side copies into d_base / d_ex must not be overwritten while
+      // the graph may still read them: keep one segment in flight
+      if ((e = hipStreamSynchronize(s)) != hipSuccess) {
+        st = hip_fail(e, "hipStreamSynchronize");
+        break;
+      }
+      step += seg;
+      continue;
+    }
+    // eager path (timed runs, ragged tails, jobs with different lengths)
+    std::vector<int> act;
+    for (int k = 0; k < n_jobs; ++k)
+      if (step < mc[k].steps_total) act.push_back(k);
+    if (act.size() != (size_t)B.n_jobs) {
+      // shrink the lists to the jobs still running; their step0 stays 0
+      if ((st = B.build(jobs, act, s)) != GD_OK) break;
+    }
+    ex.clear();
+    for (int q = 0; q < (int)act.size(); ++q) {
+      const gd_mc_param &m = mc[act[q]];
+      if (m.steps_per_exchange > 0 && jobs[act[q]]->h.n_tmp > 1 && (step + 1) % m.steps_per_exchange == 0)
+        ex.push_back({q, (step + 1) / m.steps_per_exchange - 1});
+    }
+    st = batch_step(B, lc, grid, step, nullptr, &ex, s, stats ? &tp : nullptr);
+    if (st != GD_OK) break;
+    if (!ex.empty() || (stats && (step + 1) % 64 == 0)) {
+      hipError_t e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        st = hip_fail(e, "hipStreamSynchronize");
+        break;
+      }
+      if (stats && (st = tp.collect()) != GD_OK) break;
+    }
+    ++step;
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  if (st == GD_OK && e != hipSuccess) st = hip_fail(e, "hipStreamSynchronize");
+  if (st == GD_OK && stats) st = tp.collect();
+  const auto t1 = std::chrono::steady_clock::now();
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (st != GD_OK) return st;
+  if (stats) {
+    stats->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    stats->ms_pair_kernel = tp.ms;
+    stats->ms_finish_kernel = 0.0;
+    stats->n_pair_launch = tp.n;
+    stats->pair_evals = pair_evals;
+    stats->replica_steps = rep_steps;
+  }
+  if (out)
+    for (int k = 0; k < n_jobs; ++k)
+      if ((st = read_out(jobs[k], &out[k])) != GD_OK) return st;
+  return GD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: sweep ligand atoms per work item x protein tile size on the
+// job itself for `steps` MC steps each, keep the fastest (HIP-event time of
+// the pair kernel per pair evaluation), cache it by size bucket.
+namespace {
+std::mutex g_tune_mu;
+std::map<long long, std::pair<int, int>> g_tune_cache;  // bucket -> (lc, ptile)
+long long tune_bucket(int L, int P, int R) {
+  auto lg = [](int x) {
+    int b = 0;
+    while ((1 << (b + 1)) <= x) ++b;
+    return b;
+  };
+  return ((long long)lg(L) << 40) | ((long long)lg(P) << 20) | lg(std::max(1, R));
+}
+}  // namespace
+
+namespace gdr {
+bool tuned_tiling(int L, int P, int R, int &lc, int &ptile) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune_cache.find(tune_bucket(L, P, R));
+  if (it == g_tune_cache.end()) return false;
+  lc = it->second.first;
+  ptile = std::min(P, it->second.second);
+  return true;
+}
+}  // namespace gdr
+
+extern "C" gd_status gd_autotune(gd_job *j, int steps, gd_tiling *best, double *best_ms) {
+  if (!j || steps <= 0) return GD_ERR_ARG;
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  const GdJob g0 = j->h;
+  const int L = g0.L, P = g0.P;
+  std::vector<std::pair<int, int>> cand;
+  for (int lc : {4, 8, 16}) {
+    if (lc == 16 && L < 16) continue;
+    for (int pt : {128, 256, 512, 1024, 2048})
+      if (pt <= std::max(128, P)) cand.push_back({lc, std::min(pt, P)});
+    int dlc, dpt;
+    default_tiling(L, P, g0.R, dlc, dpt);
+    cand.push_back({lc, dpt});
+  }
+  gd_mc_param mc;
+  std::memset(&mc, 0, sizeof mc);
+  mc.steps_total = steps;
+  mc.steps_per_dump = steps;
+  mc.move_mode = GD_MOVE_GAUSSIAN;
+  for (int i = 0; i < 6; ++i) mc.move_scale[i] = i < 3 ? 0.2f : 0.05f;
+  mc.seed = 1;
+  double best_t = 1e300;
+  std::pair<int, int> bestc = {g0.lc, g0.ptile};
+  for (auto &c : cand) {
+    if ((st = apply_tiling(j, c.first, c.second)) != GD_OK) return st;
+    if ((st = sync_job_struct(j)) != GD_OK) return st;
+    gd_stats s;
+    if ((st = gd_dock(j, &mc, nullptr, &s)) != GD_OK) return st;
+    const double t = s.n_pair_launch ? s.ms_pair_kernel / s.n_pair_launch : 1e300;
+    if (t < best_t) {
+      best_t = t;
+      bestc = c;
+    }
+  }
+  if ((st = apply_tiling(j, bestc.first, bestc.second)) != GD_OK) return st;
+  if ((st = sync_job_struct(j)) != GD_OK) return st;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[tune_bucket(L, P, g0.R)] = bestc;
+  }
+  if (best) {
+    best->atoms_per_lane = bestc.first;
+    best->tile_points = bestc.second;
+    best->waves_per_cu = 16;
+  }
+  if (best_ms) *best_ms = best_t;
+  return GD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Serving loop: requests queue on the host, are admitted when their device
+// footprint fits the HBM budget and an active slot is free, run in shared
+// launches with every other active job (continuous batching), and retire
+// as soon as their own step count is reached.
+struct ServJob {
+  int64_t ticket;
+  gd_job_desc desc;
+  gd_mc_param mc;
+  gd_replica_out *out;
+  size_t foot;
+  gd_job *job = nullptr;
+  int done_steps = 0;
+  int step0 = 0;
+  double t_submit = 0, t_admit = 0;
+  bool init_done = false;
+};
+
+struct gd_server {
+  gd_server_config cfg;
+  std::deque<ServJob *> pending;
+  std::vector<ServJob *> active;
+  std::vector<gd_job_result> results;
+  size_t used = 0;
+  int64_t next_ticket = 1;
+  int launch_step = 0;
+  hipStream_t stream = nullptr;
+  BatchLists lists;
+  bool dirty = true;
+  int lc = 8;
+  std::chrono::steady_clock::time_point t0;
+  std::mutex mu;
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace gdr {
+size_t footprint(const gd_job_desc *d);
+}
+
+extern "C" gd_status gd_server_create(const gd_server_config *cfg, gd_server **out) {
+  if (!cfg || !out || cfg->sweep_steps <= 0 || cfg->max_active <= 0) return GD_ERR_ARG;
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  gd_server *s = new gd_server();
+  s->cfg = *cfg;
+  s->t0 = std::chrono::steady_clock::now();
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  *out = s;
+  return GD_OK;
+}
+
+extern "C" gd_status gd_server_destroy(gd_server *s) {
+  if (!s) return GD_OK;
+  (void)hipStreamSynchronize(s->stream);
+  for (ServJob *j : s->active) {
+    gd_job_destroy(j->job);
+    delete j;
+  }
+  for (ServJob *j : s->pending) delete j;
+  (void)hipStreamDestroy(s->stream);
+  delete s;
+  return GD_OK;
+}
+
+extern "C" double gd_server_clock(gd_server *s) { return s ? s->now() : 0.0; }
+
+extern "C" gd_status gd_server_submit(gd_server *s, const gd_job_desc *desc, const gd_mc_param *mc,
+                                      gd_replica_out *out, int64_t *ticket) {
+  if (!s || !desc || !mc || mc->steps_total < 0) return GD_ERR_ARG;
+  const size_t foot = footprint(desc);
+  if (foot == 0) return GD_ERR_ARG;
+  if (foot > s->cfg.hbm_budget) return GD_ERR_BUSY;  // can never be admitted
+  std::lock_guard<std::mutex> lk(s->mu);
+  ServJob *j = new ServJob();
+  j->ticket = s->next_ticket++;
+  j->desc = *desc;
+  j->mc = *mc;
+  j->out = out;
+  j->foot = foot;
+  j->t_submit = s->now();
+  s->pending.push_back(j);
+  if (ticket) *ticket = j->ticket;
+  return GD_OK;
+}
+
+extern "C" gd_status gd_server_cancel(gd_server *s, int64_t ticket) {
+  if (!s) return GD_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (auto it = s->pending.begin(); it != s->pending.end(); ++it)
+    if ((*it)->ticket == ticket) {
+      s->results.push_back({ticket, GD_ERR_STATE, (*it)->t_submit, 0.0, s->now(), 0.0});
+      delete *it;
+      s->pending.erase(it);
+      return GD_OK;
+    }
+  for (auto it = s->active.begin(); it != s->active.end(); ++it)
+    if ((*it)->ticket == ticket) {
+      (void)hipStreamSynchronize(s->stream);
+      ServJob *j = *it;
+      s->results.push_back({ticket, GD_ERR_STATE, j->t_submit, j->t_admit, s->now(),
+                            (double)j->job->h.R * j->done_steps});
+      s->used -= j->foot;
+      gd_job_destroy(j->job);
+      delete j;
+      s->active.erase(it);
+      s->dirty = true;
+      return GD_OK;
+    }
+  return GD_ERR_ARG;
+}
+
+extern "C" gd_status gd_server_usage(gd_server *s, size_t *used, int *n_pending) {
+  if (!s) return GD_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (used) *used = s->used;
+  if (n_pending) *n_pending = (int)s->pending.size();
+  return GD_OK;
+}
+
+extern "C" gd_status gd_server_results(gd_server *s, gd_job_result *res, int cap, int *n) {
+  if (!s || !n || (cap > 0 && !res)) return GD_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int m = std::min(cap, (int)s->results.size());
+  for (int i = 0; i < m; ++i) res[i] = s->results[i];
+  s->results.erase(s->results.begin(), s->results.begin() + m);
+  *n = m;
+  return GD_OK;
+}
+
+extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
+  if (!s) return GD_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  hipStream_t hs = s->stream;
+  // (1) admission: FIFO while the footprint fits and a slot is free
+  std::vector<ServJob *> admitted;
+  while (!s->pending.empty() && (int)s->active.size() < s->cfg.max_active &&
+         s->used + s->pending.front()->foot <= s->cfg.hbm_budget) {
+    ServJob *j = s->pending.front();
+    s->pending.pop_front();
+    st = gd_job_create(&j->desc, &j->job);
+    if (st != GD_OK) {
+      s->results.push_back({j->ticket, st, j->t_submit, s->now(), s->now(), 0.0});
+      delete j;
+      continue;
+    }
+    if (j->job->h.lc != s->lc) {
+      if ((st = apply_tiling(j->job, s->lc, j->job->h.ptile)) != GD_OK ||
+          (st = sync_job_struct(j->job)) != GD_OK) {
+        s->results.push_back({j->ticket, st, j->t_submit, s->now(), s->now(), 0.0});
+        gd_job_destroy(j->job);
+        delete j;
+        continue;
+      }
+    }
+    set_mc(j->job->h, &j->mc);
+    j->job->h.record = 0;
+    if ((st = reset_replicas(j->job)) != GD_OK) return st;
+    j->t_admit = s->now();
+    j->step0 = s->launch_step;
+    j->job->h.step0 = s->launch_step;
+    s->used += j->foot;
+    s->active.push_back(j);
+    admitted.push_back(j);
+    s->dirty = true;
+  }
+  // (2) initial-energy pass of the newly admitted jobs, in one batch
+  if (!admitted.empty()) {
+    std::vector<gd_job *> nj;
+    std::vector<int> idx;
+    for (ServJob *j : admitted) {
+      idx.push_back((int)nj.size());
+      nj.push_back(j->job);
+    }
+    BatchLists B;
+    if ((st = B.build(nj, idx, hs)) != GD_OK) return st;
+    if ((st = batch_init(B, s->lc, pair_grid(16), hs)) != GD_OK) return st;
+    hipError_t e = hipStreamSynchronize(hs);
+    if (e != hipSuccess) return hip_fail(e, "init pass");
+  }
+  // (3) advance every active job by up to sweep_steps
+  for (int k = 0; k < s->cfg.sweep_steps && st == GD_OK; ++k) {
+    std::vector<int> act;
+    std::vector<gd_job *> jobs;
+    for (ServJob *j : s->active) jobs.push_back(j->job);
+    for (int q = 0; q < (int)s->active.size(); ++q)
+      if (s->active[q]->done_steps < s->active[q]->mc.steps_total) act.push_back(q);
+    if (act.empty()) break;
+    if (s->dirty || (int)act.size() != s->lists.n_jobs) {
+      for (int q : act) jobs[q]->h.step0 = s->active[q]->step0;
+      if ((st = s->lists.build(jobs, act, hs)) != GD_OK) return st;
+      s->dirty = false;
+    }
+    std::vector<GdExch> ex;
+    for (int q = 0; q < (int)act.size(); ++q) {
+      ServJob *j = s->active[act[q]];
+      const int spe = j->mc.steps_per_exchange;
+      if (spe > 0 && j->job->h.n_tmp > 1 && (j->done_steps + 1) % spe == 0)
+        ex.push_back({q, (j->done_steps + 1) / spe - 1});
+    }
+    st = batch_step(s->lists, s->lc, pair_grid(16), s->launch_step, nullptr, &ex, hs, nullptr);
+    if (!ex.empty()) (void)hipStreamSynchronize(hs);  // ex table is reused next step
+    ++s->launch_step;
+    for (int q : act) s->active[q]->done_steps += 1;
+    // jobs whose step counts end here leave the lists at the next step
+    bool any_done = false;
+    for (int q : act)
+      any_done = any_done || s->active[q]->done_steps >= s->active[q]->mc.steps_total;
+    if (any_done) s->dirty = true;
+  }
+  hipError_t e = hipStreamSynchronize(hs);
+  if (st == GD_OK && e != hipSuccess) st = hip_fail(e, "sweep");
+  if (st != GD_OK) return st;
+  // (4) retire finished jobs
+  for (auto it = s->active.begin(); it != s->active.end();) {
+    ServJob *j = *it;
+    if (j->done_steps >= j->mc.steps_total) {
+      gd_status rs = read_out(j->job, j->out);
+      s->results.push_back({j->ticket, rs, j->t_submit, j->t_admit, s->now(),
+                            (double)j->job->h.R * j->mc.steps_total});
+      s->used -= j->foot;
+      gd_job_destroy(j->job);
+      delete j;
+      it = s->active.erase(it);
+      s->dirty = true;
+    } else {
+      ++it;
+    }
+  }
+  if (n_active) *n_active = (int)s->active.size();
+  if (n_done) *n_done = (int)s->results.size();
+  return GD_OK;
+}

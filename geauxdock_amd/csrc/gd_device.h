@@ -1,0 +1,109 @@
+// gd_device.h -- device-resident layout of a docking job, shared by the
+// kernels (gd_kernels.hip) and the host runtime (gd_runtime.hip).
+//
+// MI355X-first layout rules:
+//  * replicas are the SIMD lanes: every per-replica array is SoA over a
+//    replica index padded to a multiple of 64 (Rp), so a wave touches one
+//    contiguous 256-byte line per field;
+//  * everything shared by the replicas of a job (force-field rows, protein
+//    point attributes, KDE anchors, MCS targets) is read through the scalar
+//    data cache with wave-uniform addresses -- one fetch serves 64 lanes;
+//  * only the protein coordinates depend on a lane (its protein conformer):
+//    float4 [n_prt][P], <= a handful of distinct rows per wave;
+//  * partial sums are [slot][Rp] so both the pair kernel (writer) and the
+//    finish kernel (reader) are coalesced across lanes.
+#pragma once
+#include <stdint.h>
+
+#define GD_WAVE 64
+#define GD_PTAB_W 8       // floats per (ligand type, point type) entry
+#define GD_PTAB_PT 32     // point-type stride of the table (31 used)
+#define GD_REC_W 17       // floats per accept record: step, movematrix[6], energy[10]
+#define GD_NLIN 6         // linear partial sums: vdw ele pmf psp hdb kde
+
+// point flag bits (pattr.z as int)
+#define GD_PF_TYPE 0x1f
+#define GD_PF_CDC 0x100
+#define GD_PF_CLS2 0x200
+
+// finish-kernel modes
+#define GD_MODE_INIT 0    // initial-energy pass: zero move, forced accept
+#define GD_MODE_STEP 1    // MC step: Metropolis with the replica's draw
+#define GD_MODE_EVAL 2    // energy only, results to the eval buffers
+
+struct GdAtom {           // per ligand atom, type-sorted order
+  int tidx;               // row of the job's parameter table
+  float q;                // partial charge
+  int kde_b, kde_e;       // KDE anchors of the atom's type [b, e)
+  float kde_inv_n;        // 1 / (e - b), 0 if empty
+  float hpl0, hpl1, hpl2; // hydrophobic restraint of the atom's type
+};
+
+struct GdJob {
+  // sizes and tiling
+  int L, P, K, M;
+  int n_lig, n_prt, n_tmp, R, Rp;
+  int nL, nP, ptile;      // ligand chunks, protein tiles, points per tile
+  int lc;                 // ligand atoms per chunk (kernel template value)
+  int n_types;            // rows of the parameter table
+  // constant tables (device)
+  const float4 *lig0;     // [n_lig][L] relative conformer coordinates
+  const GdAtom *atoms;    // [L]
+  const float4 *pcoord;   // [n_prt][P]
+  const float4 *pattr;    // [P] (ele, hpp, flags-as-int, 0)
+  const float *ptab;      // [n_types][GD_PTAB_PT][GD_PTAB_W]
+  const float *pspv;      // [n_types][P] psp value of class-2 points
+  const float4 *kde;      // [K] sorted by type
+  const float4 *mcs;      // [M][L] (x, y, z, valid)
+  const float2 *mcsrow;   // [M] (tcc, n_valid)
+  const float *minus_beta;// [n_tmp]
+  // replica state (device, SoA over Rp)
+  float *mm;              // [6][Rp] accepted movematrix
+  float *prop;            // [6][Rp] proposed movematrix of the current step
+  float *en;              // [10][Rp] accepted energies
+  int *ligi, *prti, *tmpi;// [Rp] conformer / temperature indices
+  int *acc;               // [Rp] last step accepted
+  int *nacc, *nexc;       // [Rp] counters
+  float *best;            // [Rp]
+  int *own;               // [n_prt*n_lig][n_tmp] replica at each temperature
+  float *coords;          // [L][3][Rp] transformed ligand of the current proposal
+  float *part_lin;        // [GD_NLIN][nL*nP][Rp]
+  float *part_hpc;        // [L][nP][Rp]
+  float *part_mcs;        // [M][nL][Rp]
+  float *rec;             // [Rp][rec_cap][GD_REC_W] accept records (optional)
+  int *rec_n;             // [Rp]
+  int rec_cap;
+  float *ev_terms;        // [Rp][10] eval outputs
+  float *ev_raw;          // [Rp][9]
+  // scalars
+  float lj0, lj1, el0, el1, a1, b1, inv_el1;
+  float kde2l;            // kde2 * log2(e)
+  float kde3_inv;
+  float inv_L;
+  float center[3], pocket[3];
+  float a_para[10], b_para[10], w[10];
+  float move_scale[6];
+  int move_mode;
+  uint32_t seed_lo, seed_hi, job_id;
+  int record;
+  int step0;              // launch step at which this job's MC step 0 ran
+  int steps_total;
+  int *wq;                // work-queue counter of the persistent pair kernel
+};
+
+struct GdItem {           // one wave of the pair kernel
+  int job;
+  int rb;                 // first replica of the 64-lane block
+  int a;                  // ligand chunk
+  int b;                  // protein tile
+};
+
+struct GdBlock {          // one workgroup of the finish / transform kernels
+  int job;
+  int rb;
+};
+
+struct GdExch {           // one workgroup of the exchange kernel
+  int job;
+  int round;
+};

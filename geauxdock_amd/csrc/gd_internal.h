@@ -1,0 +1,60 @@
+// gd_internal.h -- host-side job object and runtime helpers shared by the
+// runtime translation units (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/geauxdock_amd.h"
+#include "gd_device.h"
+#include "gd_rng.h"
+
+struct gd_job {
+  GdJob h{};                 // host mirror of the device struct
+  GdJob *d_job = nullptr;    // device copy
+  void *d_base = nullptr;    // tables + replica state
+  void *d_part = nullptr;    // partial sums (tiling-dependent)
+  void *d_lists = nullptr;   // work items + blocks
+  void *d_rec = nullptr;     // accept records
+  size_t rec_bytes = 0;
+  size_t bytes = 0;          // d_base size
+  std::vector<GdItem> items; // job-local work items (job index 0)
+  GdItem *d_items = nullptr;
+  GdBlock *d_blocks = nullptr;
+  int n_items = 0, n_blocks = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace gdr {
+void set_error(const std::string &s);
+gd_status hip_fail(hipError_t e, const char *what);
+gd_status ensure_device();
+int n_cu();
+void default_tiling(int L, int P, int R, int &lc, int &ptile);
+gd_status apply_tiling(gd_job *j, int lc, int ptile);
+gd_status sync_job_struct(gd_job *j);
+gd_status reset_replicas(gd_job *j);
+void set_mc(GdJob &g, const gd_mc_param *mc);
+gd_status read_out(gd_job *j, gd_replica_out *out);
+gd_status launch_pair(int lc, const GdJob *d_jobs, const GdItem *d_items, int n_items, hipStream_t s);
+gd_status launch_finish(const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks, int mode,
+                        int step, hipStream_t s);
+gd_status launch_transform(const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks, hipStream_t s);
+gd_status launch_exchange(const GdJob *d_jobs, const GdExch *d_ex, int n_ex, hipStream_t s);
+
+// HIP-event timer for one kernel over many launches: events are recorded
+// around every launch and summed at synchronisation points, so timing never
+// stalls the launch queue.
+struct KTimer {
+  std::vector<hipEvent_t> ev;
+  int used = 0;
+  double ms = 0.0;
+  int n = 0;
+  ~KTimer();
+  gd_status begin(hipStream_t s);   // records start event
+  gd_status end(hipStream_t s);     // records stop event
+  gd_status collect();              // after a stream sync
+};
+}  // namespace gdr

@@ -1,0 +1,531 @@
+// gd_kernels.hip -- hand-written gfx950 kernels of the Monte Carlo hot path.
+//
+// One MC step of every replica is two launches:
+//   gd_pair_kernel   one wave per work item (64 replicas x one ligand-atom
+//                    chunk x one protein tile): PRT pair terms, the item's
+//                    slice of the KDE anchors and of the MCS rows; writes
+//                    partial sums.  Lanes are replicas, so protein points,
+//                    force-field rows, KDE anchors and MCS targets are
+//                    wave-uniform scalar loads and the inner loop is pure FP32
+//                    VALU on ligand coordinates held in VGPRs.
+//   gd_finish_kernel one workgroup per 64-replica block: reduces the partials
+//                    (waves split the slots, LDS combines them), applies the
+//                    non-linear per-atom/per-row terms, normalisation and
+//                    weights, the Metropolis test, records, and proposes +
+//                    transforms the next move (Philox4x32-10, counter-based).
+// plus gd_transform_kernel (pose -> ligand coordinates, used before the first
+// pass and for energy evaluation) and gd_exchange_kernel (temperature swaps).
+//
+// The reference computes the same quantities in one CUDA block per replica
+// (backend_gpu/kernel_cuda_l1_montecarlo.cu:29-536); here the pair loop
+// formulas follow backend_cpu_mic/energy_prt_v1.C:30-134, the KDE term
+// energy_kde_v1.C:1-51, the MCS term energy_mcs_v1.C:1-31 and the
+// step/accept logic kernel_cpp_montecarlo_soa.C:219-413.
+#include <hip/hip_runtime.h>
+
+#include "gd_device.h"
+#include "gd_rng.h"
+
+#define GD_LOG2E 1.44269504088896340736f
+// Smallest float x with expf(x) > 0 in IEEE arithmetic with subnormals: the
+// reference's forced initial accept (u = 0 < expf(dE * beta),
+// kernel_cpp_montecarlo_soa.C:384-385) fails below it.
+#define GD_EXP_UNDERFLOW (-103.97207641601562f)
+
+__device__ __forceinline__ int gd_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float gd_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float gd_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float gd_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// hydrophobic polynomial 1 - (a r^2 - b r^4 + c r^6 - d r^8), r^2 in A^2
+#define GD_HPA (3.5f / 81.0f)
+#define GD_HPB (4.5f / 81.0f / 81.0f)
+#define GD_HPC (2.5f / 81.0f / 81.0f / 81.0f)
+#define GD_HPD (0.5f / 81.0f / 81.0f / 81.0f / 81.0f)
+
+// ---------------------------------------------------------------------------
+// Address-space helpers: tables shared by all lanes are read through the
+// constant address space, which with a wave-uniform address compiles to
+// s_load (scalar data cache, one fetch for 64 lanes); per-lane data goes
+// through the global address space (global_load/store, not flat).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GD_AS_CONST __attribute__((address_space(4)))
+#define GD_AS_GLOBAL __attribute__((address_space(1)))
+#else  // host pass of this translation unit: kernels are only declared there
+#define GD_AS_CONST
+#define GD_AS_GLOBAL
+#endif
+template <class T>
+__device__ __forceinline__ const GD_AS_CONST T *gd_c(const T *p) {
+  return (const GD_AS_CONST T *)p;
+}
+template <class T>
+__device__ __forceinline__ GD_AS_GLOBAL T *gd_g(T *p) {
+  return (GD_AS_GLOBAL T *)p;
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+__device__ __forceinline__ const GD_AS_GLOBAL T *gd_g(const T *p) {
+  return (const GD_AS_GLOBAL T *)p;
+}
+#endif
+
+// ---------------------------------------------------------------------------
+// Pair kernel.  LC ligand atoms per wave, all LC coordinates in registers.
+template <int LC>
+__device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, const GdItem it,
+                                             const int lane) {
+  const GD_AS_CONST GdJob &J = gd_c(jobs)[it.job];
+  const int Rp = J.Rp, P = J.P, L = J.L, nP = J.nP;
+  const int r = it.rb + lane;  // < Rp by construction (Rp padded to 64)
+  const int l0 = it.a * LC;
+  const int nl = min(LC, L - l0);
+  const int p0 = it.b * J.ptile;
+  const int p1 = min(P, p0 + J.ptile);
+  const GD_AS_CONST float4 *__restrict__ pattr = gd_c(J.pattr);
+  const GD_AS_CONST float *__restrict__ ptab = gd_c(J.ptab);
+  const GD_AS_CONST float *__restrict__ pspv = gd_c(J.pspv);
+  const GD_AS_CONST GdAtom *__restrict__ atoms = gd_c(J.atoms);
+  const GD_AS_GLOBAL float *__restrict__ coords = gd_g(J.coords);
+
+  // ligand chunk: coordinates of this lane's replica, per-atom constants (uniform)
+  float lx[LC], ly[LC], lz[LC], ele[LC], hpc[LC];
+  int tid[LC];
+#pragma unroll
+  for (int j = 0; j < LC; ++j) {
+    const int l = l0 + min(j, nl - 1);
+    lx[j] = coords[(size_t)(3 * l + 0) * Rp + r];
+    ly[j] = coords[(size_t)(3 * l + 1) * Rp + r];
+    lz[j] = coords[(size_t)(3 * l + 2) * Rp + r];
+    tid[j] = atoms[l].tidx * GD_PTAB_PT;
+    ele[j] = 0.0f;
+    hpc[j] = 0.0f;
+  }
+  const GD_AS_GLOBAL float4 *__restrict__ pc = gd_g(J.pcoord) + (size_t)gd_g(J.prti)[r] * P;
+  const float lj0 = J.lj0, lj1 = J.lj1, el0 = J.el0, el1 = J.el1, a1 = J.a1, b1 = J.b1;
+  const float inv_el1 = J.inv_el1;
+  float evdw = 0.0f, epmf = 0.0f, epsp = 0.0f, ehdb = 0.0f;
+
+  float4 cn = pc[p0 < p1 ? p0 : 0];
+  for (int p = p0; p < p1; ++p) {
+    const float4 c = cn;
+    if (p + 1 < p1) cn = pc[p + 1];
+    const float4 pa = pattr[p];
+    const int fl = __float_as_int(pa.z);
+    const int pt = fl & GD_PF_TYPE;
+    const float pele = pa.x, phpp = pa.y;
+#pragma unroll
+    for (int j = 0; j < LC; ++j) {
+      if (j < nl) {
+        const GD_AS_CONST float *__restrict__ prm = ptab + (size_t)(tid[j] + pt) * GD_PTAB_W;
+        const float p1a = prm[0], p2a = prm[1], pmf0 = prm[2], pmf1 = prm[3];
+        const float hdb0 = prm[4], hdb1c = prm[5], hdb1 = prm[6], pthr = prm[7];
+        const float dx = lx[j] - c.x, dy = ly[j] - c.y, dz = lz[j] - c.z;
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const float ri = gd_rsq(r2);
+        const float rr = r2 * ri;
+        // hydrophobic contact (energy_prt_v1.C:44-50)
+        if (fl & GD_PF_CDC) {
+          float t = __builtin_fmaf(-GD_HPD, r2, GD_HPC);
+          t = __builtin_fmaf(t, r2, -GD_HPB);
+          t = __builtin_fmaf(t, r2, GD_HPA);
+          t *= r2;
+          const float h = __builtin_fmaf(-phpp, t, phpp);
+          hpc[j] += (r2 <= 81.0f) ? h : 0.0f;
+        }
+        // soft Lennard-Jones (:61-64) with 1/r^6, 1/r^9 from one rsq
+        {
+          const float ri2 = ri * ri;
+          const float ri3 = ri2 * ri;
+          const float ri6 = ri3 * ri3;
+          const float ri9 = ri6 * ri3;
+          const float q1 = p1a * ri9;
+          const float q2 = p2a * ri6;
+          const float den = __builtin_fmaf(q1 * lj0, __builtin_fmaf(lj1, r2, 1.0f), 1.0f);
+          evdw = __builtin_fmaf(q1 - q2, gd_rcp(den), evdw);
+        }
+        // distance-dependent electrostatics (:72-78), charge applied per atom
+        {
+          const float s = el1 * rr;
+          const float g = (s < 1.0f) ? __builtin_fmaf(__builtin_fmaf(b1, s, a1), s * s, el0)
+                                     : ri * inv_el1;
+          ele[j] = __builtin_fmaf(pele, g, ele[j]);
+        }
+        // contact PMF (:88-91)
+        const float dmp = rr - pmf0;
+        {
+          const float arg = __builtin_fmaf(-0.5f * GD_LOG2E, rr, 6.0f * GD_LOG2E) * dmp;
+          epmf = __builtin_fmaf(pmf1, gd_rcp(1.0f + gd_exp2(arg)), epmf);
+        }
+        // pocket-specific potential (:100-107): r <= pmf0 decided on r^2
+        // against the exact float threshold of sqrtf(r^2) <= pmf0
+        if (fl & GD_PF_CLS2) {
+          const float v = pspv[(size_t)(tid[j] / GD_PTAB_PT) * P + p];
+          epsp += (r2 <= pthr) ? v : 0.0f;
+        }
+        // hydrogen bond (:116-121); rows with hdb0 <= 0.1 carry hdb1 == 0
+        if (__float_as_int(hdb1) != 0) {
+          const float h = (rr - hdb0) * hdb1c;
+          ehdb = __builtin_fmaf(hdb1, gd_exp2(-(h * h)), ehdb);
+        }
+      }
+    }
+  }
+
+  // KDE: this tile's slice of every chunk atom's anchor bin
+  float ekde = 0.0f;
+  const float kde2l = J.kde2l;
+  const GD_AS_CONST float4 *__restrict__ kde = gd_c(J.kde);
+#pragma unroll
+  for (int j = 0; j < LC; ++j) {
+    if (j < nl) {
+      const GdAtom at = atoms[l0 + j];
+      const int nb = at.kde_e - at.kde_b;
+      const int k0 = at.kde_b + (int)(((long long)nb * it.b) / nP);
+      const int k1 = at.kde_b + (int)(((long long)nb * (it.b + 1)) / nP);
+      float s = 0.0f;
+      for (int k = k0; k < k1; ++k) {
+        const float4 kp = kde[k];
+        const float dx = lx[j] - kp.x, dy = ly[j] - kp.y, dz = lz[j] - kp.z;
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        s += gd_exp2(kde2l * d2);
+      }
+      ekde = __builtin_fmaf(s, at.kde_inv_n, ekde);
+    }
+  }
+
+  // MCS: this tile's slice of the rows, restricted to the chunk atoms
+  {
+    const int M = J.M, nL = J.nL;
+    const GD_AS_CONST float4 *__restrict__ mcs = gd_c(J.mcs);
+    GD_AS_GLOBAL float *__restrict__ pm = gd_g(J.part_mcs);
+    const int m0 = (int)(((long long)M * it.b) / nP);
+    const int m1 = (int)(((long long)M * (it.b + 1)) / nP);
+    for (int m = m0; m < m1; ++m) {
+      float d = 0.0f;
+#pragma unroll
+      for (int j = 0; j < LC; ++j) {
+        if (j < nl) {
+          const float4 t = mcs[(size_t)m * L + l0 + j];
+          if (t.w != 0.0f) {
+            const float dx = lx[j] - t.x, dy = ly[j] - t.y, dz = lz[j] - t.z;
+            d += __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+          }
+        }
+      }
+      pm[((size_t)m * nL + it.a) * Rp + r] = d;
+    }
+  }
+
+  // partial sums
+  float eele = 0.0f;
+#pragma unroll
+  for (int j = 0; j < LC; ++j)
+    if (j < nl) eele = __builtin_fmaf(atoms[l0 + j].q, ele[j], eele);
+  const int nS = J.nL * nP;
+  const int slot = it.a * nP + it.b;
+  GD_AS_GLOBAL float *__restrict__ pl = gd_g(J.part_lin) + (size_t)slot * Rp + r;
+  const size_t kst = (size_t)nS * Rp;
+  pl[0 * kst] = evdw;
+  pl[1 * kst] = eele;
+  pl[2 * kst] = epmf;
+  pl[3 * kst] = epsp;
+  pl[4 * kst] = ehdb;
+  pl[5 * kst] = ekde;
+  GD_AS_GLOBAL float *__restrict__ ph = gd_g(J.part_hpc);
+#pragma unroll
+  for (int j = 0; j < LC; ++j)
+    if (j < nl) ph[((size_t)(l0 + j) * nP + it.b) * Rp + r] = hpc[j];
+}
+
+// One wave per work item (grid = items / 4 workgroups of 4 waves).
+template <int LC>
+__global__ __launch_bounds__(256) void gd_pair_kernel(const GdJob *__restrict__ jobs,
+                                                       const GdItem *__restrict__ items,
+                                                       int n_items) {
+  const int iid = gd_uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (iid >= n_items) return;
+  gd_pair_item<LC>(jobs, gd_c(items)[iid], threadIdx.x & 63);
+}
+
+// Persistent form: a fixed grid sized to the machine; every wave pulls item
+// indices from a device counter until the list is exhausted, which balances
+// items of uneven cost (complexes of different sizes in one launch).  The
+// counter is zeroed by the kernel that runs before each launch (finish or
+// transform kernel); every wave exits once the counter passes n_items.
+template <int LC>
+__global__ __launch_bounds__(256) void gd_pair_kernel_wq(const GdJob *__restrict__ jobs,
+                                                          const GdItem *__restrict__ items,
+                                                          int n_items, int *__restrict__ wq) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int iid = 0;
+    if (lane == 0) iid = atomicAdd(wq, 1);
+    iid = gd_uniform(__shfl(iid, 0));
+    if (iid >= n_items) break;
+    gd_pair_item<LC>(jobs, gd_c(items)[iid], lane);
+  }
+}
+
+template __global__ void gd_pair_kernel<4>(const GdJob *, const GdItem *, int);
+template __global__ void gd_pair_kernel<8>(const GdJob *, const GdItem *, int);
+template __global__ void gd_pair_kernel<16>(const GdJob *, const GdItem *, int);
+template __global__ void gd_pair_kernel_wq<4>(const GdJob *, const GdItem *, int, int *);
+template __global__ void gd_pair_kernel_wq<8>(const GdJob *, const GdItem *, int, int *);
+template __global__ void gd_pair_kernel_wq<16>(const GdJob *, const GdItem *, int, int *);
+
+// ---------------------------------------------------------------------------
+// Ligand transform of one replica's proposal (kernel_cpp_montecarlo_soa.C:
+// 256-293): waves of the block split the atoms, lanes are replicas.
+__device__ __forceinline__ void gd_rotation(const float m[6], float R[9]) {
+  float s1, c1, s2, c2, s3, c3;
+  sincosf(m[3], &s1, &c1);
+  sincosf(m[4], &s2, &c2);
+  sincosf(m[5], &s3, &c3);
+  R[0] = c1 * c2;
+  R[1] = c1 * s2 * s3 - c3 * s1;
+  R[2] = s1 * s3 + c1 * c3 * s2;
+  R[3] = c2 * s1;
+  R[4] = c1 * c3 + s1 * s2 * s3;
+  R[5] = c3 * s1 * s2 - c1 * s3;
+  R[6] = -s2;
+  R[7] = c2 * s3;
+  R[8] = c2 * c3;
+}
+
+__device__ __forceinline__ void gd_transform_atoms(const GdJob &J, int r, const float R[9],
+                                                   const float t[3], int wave, int nwave) {
+  const int L = J.L, Rp = J.Rp;
+  const float4 *lc = J.lig0 + (size_t)J.ligi[r] * L;
+  for (int l = wave; l < L; l += nwave) {
+    const float4 a = lc[l];
+    J.coords[(size_t)(3 * l + 0) * Rp + r] = R[0] * a.x + R[1] * a.y + R[2] * a.z + t[0];
+    J.coords[(size_t)(3 * l + 1) * Rp + r] = R[3] * a.x + R[4] * a.y + R[5] * a.z + t[1];
+    J.coords[(size_t)(3 * l + 2) * Rp + r] = R[6] * a.x + R[7] * a.y + R[8] * a.z + t[2];
+  }
+}
+
+// Transform the current proposal (prop) of every replica in the block.
+__global__ __launch_bounds__(256) void gd_transform_kernel(const GdJob *__restrict__ jobs,
+                                                            const GdBlock *__restrict__ blocks,
+                                                            int *__restrict__ wq) {
+  if (wq && blockIdx.x == 0 && threadIdx.x == 0) *wq = 0;  // next pair launch's queue
+  const GdBlock bk = blocks[blockIdx.x];
+  const GdJob &J = jobs[bk.job];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = bk.rb + lane;
+  float m[6];
+  for (int i = 0; i < 6; ++i) m[i] = J.prop[(size_t)i * J.Rp + r];
+  float R[9];
+  gd_rotation(m, R);
+  const float t[3] = {m[0] + J.center[0], m[1] + J.center[1], m[2] + J.center[2]};
+  gd_transform_atoms(J, r, R, t, wave, 4);
+}
+
+// ---------------------------------------------------------------------------
+// Move increments for step `step` (reference: move_scale * U[0,1),
+// kernel_cpp_montecarlo_soa.C:242-250; gaussian mode: move_scale * N(0,1)).
+__device__ __forceinline__ void gd_move(const GdJob &J, int r, uint32_t step, float inc[6],
+                                        float *u_acc) {
+  uint32_t w[8];
+  gd_step_draws(((uint64_t)J.seed_hi << 32) | J.seed_lo, J.job_id, (uint32_t)r, step, w);
+  if (J.move_mode == 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float u1 = gd_u01(w[2 * k]), u2 = gd_u01(w[2 * k + 1]);
+      const float rad = sqrtf(-2.0f * logf(1.0f - u1));
+      float sn, cs;
+      sincosf(6.28318530717958647692f * u2, &sn, &cs);
+      inc[2 * k] = J.move_scale[2 * k] * (rad * cs);
+      inc[2 * k + 1] = J.move_scale[2 * k + 1] * (rad * sn);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) inc[i] = J.move_scale[i] * gd_u01(w[i]);
+  }
+  *u_acc = gd_u01(w[6]);
+}
+
+// ---------------------------------------------------------------------------
+// Finish kernel: 256 threads = 4 waves per 64-replica block.
+__global__ __launch_bounds__(256) void gd_finish_kernel(const GdJob *__restrict__ jobs,
+                                                         const GdBlock *__restrict__ blocks,
+                                                         int mode, int launch_step,
+                                                         const int *__restrict__ step_base,
+                                                         int *__restrict__ wq) {
+  if (wq && blockIdx.x == 0 && threadIdx.x == 0) *wq = 0;  // next pair launch's queue
+  if (step_base) launch_step += *step_base;  // graph replays: base step in device memory
+  __shared__ float red[4][8][64];
+  __shared__ float xf[12][64];
+  const GdBlock bk = blocks[blockIdx.x];
+  const GdJob &J = jobs[bk.job];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = bk.rb + lane;
+  const int Rp = J.Rp;
+  const int nS = J.nL * J.nP;
+  const int step = launch_step - J.step0;  // this job's MC step
+
+  // (1) slot reductions, waves interleaved over slots / atoms / rows
+  float acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < GD_NLIN; ++k) {
+    const float *src = J.part_lin + (size_t)k * nS * Rp + r;
+    float s = 0.0f;
+    for (int q = wave; q < nS; q += 4) s += src[(size_t)q * Rp];
+    acc[k] = s;
+  }
+  for (int l = wave; l < J.L; l += 4) {  // hpc per atom (energy_prt_v1.C:132-133)
+    const float *src = J.part_hpc + (size_t)l * J.nP * Rp + r;
+    float h = 0.0f;
+    for (int b = 0; b < J.nP; ++b) h += src[(size_t)b * Rp];
+    const GdAtom at = J.atoms[l];
+    const float t = (h - at.hpl0) / at.hpl1;
+    acc[6] += 0.5f * t * t - at.hpl2;
+  }
+  for (int m = wave; m < J.M; m += 4) {  // MCS rows (energy_mcs_v1.C:5-29)
+    const float *src = J.part_mcs + (size_t)m * J.nL * Rp + r;
+    float d = 0.0f;
+    for (int a = 0; a < J.nL; ++a) d += src[(size_t)a * Rp];
+    const float2 row = J.mcsrow[m];
+    acc[7] += row.x * sqrtf(d / row.y);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) red[wave][k][lane] = acc[k];
+  __syncthreads();
+
+  if (wave == 0) {
+    float s[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] = red[0][k][lane] + red[1][k][lane] + red[2][k][lane] + red[3][k][lane];
+    float m[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) m[i] = J.prop[(size_t)i * Rp + r];
+    // e_s assembly (kernel_cpp_montecarlo_soa.C:318-324), KDE/MCS/DST (:328-359)
+    float e[10];
+    e[0] = s[0];
+    e[1] = s[1];
+    e[2] = s[2];
+    e[3] = s[3];
+    e[4] = s[4] * (-1.0f / sqrtf(2.0f * 3.1415926535f));
+    e[5] = s[6];
+    e[6] = s[5] * J.kde3_inv;
+    e[7] = logf(s[7] / (float)J.M);
+    {
+      float d2 = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const float d = J.center[i] + m[i] - J.pocket[i];
+        d2 += d * d;
+      }
+      e[8] = sqrtf(d2);
+    }
+    float raw[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) raw[i] = e[i];
+    // normalisation + weights (:363-374)
+#pragma unroll
+    for (int i = 0; i < 7; ++i) e[i] = e[i] * J.inv_L;
+    float tot = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      e[i] = J.a_para[i] * e[i] + J.b_para[i];
+      tot += J.w[i] * e[i];
+    }
+    e[9] = tot;
+
+    const bool live = r < J.R;
+    if (mode == GD_MODE_EVAL) {
+      if (live) {
+        for (int i = 0; i < 10; ++i) J.ev_terms[(size_t)r * 10 + i] = e[i];
+        for (int i = 0; i < 9; ++i) J.ev_raw[(size_t)r * 9 + i] = raw[i];
+      }
+    } else {
+      // record the pre-move state if the previous step was accepted (:224-231)
+      if (mode == GD_MODE_STEP && J.record && live && J.acc[r]) {
+        const int n = J.rec_n[r];
+        if (n < J.rec_cap) {
+          float *o = J.rec + ((size_t)r * J.rec_cap + n) * GD_REC_W;
+          o[0] = (float)step;
+          for (int i = 0; i < 6; ++i) o[1 + i] = J.mm[(size_t)i * Rp + r];
+          for (int i = 0; i < 10; ++i) o[7 + i] = J.en[(size_t)i * Rp + r];
+          J.rec_n[r] = n + 1;
+        }
+      }
+      // Metropolis (:382-392); the initial pass draws u = 0
+      float u = 0.0f;
+      if (mode == GD_MODE_STEP) {
+        float inc[6];
+        gd_move(J, r, (uint32_t)step, inc, &u);
+      }
+      const float x = (tot - J.en[(size_t)9 * Rp + r]) * J.minus_beta[J.tmpi[r]];
+      const bool a = (u == 0.0f) ? (x >= GD_EXP_UNDERFLOW) : (u < expf(x));
+      if (live) {
+        J.acc[r] = a;
+        if (a) {
+          for (int i = 0; i < 10; ++i) J.en[(size_t)i * Rp + r] = e[i];
+          for (int i = 0; i < 6; ++i) J.mm[(size_t)i * Rp + r] = m[i];
+          if (mode == GD_MODE_STEP) J.nacc[r] += 1;
+          J.best[r] = fminf(J.best[r], tot);
+        }
+      }
+      // propose and transform the next step's move
+      const uint32_t next = mode == GD_MODE_STEP ? (uint32_t)step + 1u : 0u;
+      float inc[6], un;
+      gd_move(J, r, next, inc, &un);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const float v = inc[i] + J.mm[(size_t)i * Rp + r];
+        J.prop[(size_t)i * Rp + r] = v;
+        m[i] = v;
+      }
+      float R[9];
+      gd_rotation(m, R);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) xf[i][lane] = R[i];
+      xf[9][lane] = m[0] + J.center[0];
+      xf[10][lane] = m[1] + J.center[1];
+      xf[11][lane] = m[2] + J.center[2];
+    }
+  }
+  if (mode == GD_MODE_EVAL) return;
+  __syncthreads();
+  float R[9], t[3];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R[i] = xf[i][lane];
+  t[0] = xf[9][lane];
+  t[1] = xf[10][lane];
+  t[2] = xf[11][lane];
+  gd_transform_atoms(J, r, R, t, wave, 4);
+}
+
+// ---------------------------------------------------------------------------
+// Temperature exchange: one workgroup per (job, round) entry, threads over
+// the (protein conformer, ligand conformer) ladders; pairs (t, t+1) for
+// t = round mod 2, t+2, ...  Swaps temperature labels, not coordinates.
+__global__ __launch_bounds__(256) void gd_exchange_kernel(const GdJob *__restrict__ jobs,
+                                                           const GdExch *__restrict__ ex) {
+  const GdExch e = ex[blockIdx.x];
+  const GdJob &J = jobs[e.job];
+  const int nt = J.n_tmp, Rp = J.Rp;
+  const uint64_t seed = ((uint64_t)J.seed_hi << 32) | J.seed_lo;
+  for (int ladder = threadIdx.x; ladder < J.n_prt * J.n_lig; ladder += blockDim.x) {
+    int *own = J.own + (size_t)ladder * nt;
+    for (int t = e.round & 1; t + 1 < nt; t += 2) {
+      const int i = own[t], j = own[t + 1];
+      const float u =
+          gd_exchange_u01(seed, J.job_id, (uint32_t)(ladder * nt + t), (uint32_t)e.round);
+      const float x = (J.minus_beta[t + 1] - J.minus_beta[t]) *
+                      (J.en[(size_t)9 * Rp + i] - J.en[(size_t)9 * Rp + j]);
+      if (u < expf(x)) {
+        own[t] = j;
+        own[t + 1] = i;
+        J.tmpi[i] = t + 1;
+        J.tmpi[j] = t;
+        J.nexc[i] += 1;
+        J.nexc[j] += 1;
+      }
+    }
+  }
+}

@@ -1,0 +1,873 @@
+// gd_runtime.hip -- host runtime of the C ABI: device selection, packing a
+// complex into the device layout of gd_device.h, tiling, kernel launches,
+// the plain one-job docking path, energy evaluation and accept records.
+// Batching and serving build on the same job objects (gd_batch.hip).
+//
+// Reference call sequence replaced (src/backend_gpu/backend_gpu.cu:64-133,
+// prepair.C:12-135): CopyH2D of a 6.3 MiB Complex per job, one launch for
+// the initial energies, one launch + blocking D2H of records per dump.  Here
+// a job's tables are packed once into device memory sized to the job, the
+// replica state never leaves the device, and every MC step is a pair-kernel
+// + finish-kernel launch on the job's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/geauxdock_amd.h"
+#include "gd_device.h"
+#include "gd_internal.h"
+
+// kernels (gd_kernels.hip)
+template <int LC>
+__global__ void gd_pair_kernel(const GdJob *, const GdItem *, int);
+template <int LC>
+__global__ void gd_pair_kernel_wq(const GdJob *, const GdItem *, int, int *);
+__global__ void gd_transform_kernel(const GdJob *, const GdBlock *, int *);
+__global__ void gd_finish_kernel(const GdJob *, const GdBlock *, int, int, const int *, int *);
+__global__ void gd_exchange_kernel(const GdJob *, const GdExch *);
+
+namespace gdr {
+
+static thread_local std::string g_err;
+static int g_device = -1;
+static int g_cu = 0;
+static std::mutex g_mu;
+
+void set_error(const std::string &s) { g_err = s; }
+const char *last_error() { return g_err.c_str(); }
+
+gd_status hip_fail(hipError_t e, const char *what) {
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  set_error(buf);
+  return GD_ERR_HIP;
+}
+
+KTimer::~KTimer() {
+  for (auto e : ev) hipEventDestroy(e);
+}
+gd_status KTimer::begin(hipStream_t s) {
+  if (used + 2 > (int)ev.size()) {
+    hipEvent_t a, b;
+    hipError_t e;
+    if ((e = hipEventCreate(&a)) != hipSuccess || (e = hipEventCreate(&b)) != hipSuccess)
+      return hip_fail(e, "hipEventCreate");
+    ev.push_back(a);
+    ev.push_back(b);
+  }
+  hipError_t e = hipEventRecord(ev[used], s);
+  return e == hipSuccess ? GD_OK : hip_fail(e, "hipEventRecord");
+}
+gd_status KTimer::end(hipStream_t s) {
+  hipError_t e = hipEventRecord(ev[used + 1], s);
+  used += 2;
+  return e == hipSuccess ? GD_OK : hip_fail(e, "hipEventRecord");
+}
+gd_status KTimer::collect() {
+  for (int i = 0; i + 1 < used; i += 2) {
+    float t = 0.0f;
+    hipError_t e = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+    if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+    ms += t;
+    ++n;
+  }
+  used = 0;
+  return GD_OK;
+}
+
+int device() { return g_device; }
+int n_cu() { return g_cu; }
+
+gd_status ensure_device() {
+  if (g_device >= 0) {
+    hipError_t e = hipSetDevice(g_device);
+    return e == hipSuccess ? GD_OK : hip_fail(e, "hipSetDevice");
+  }
+  return gd_init(0);
+}
+
+// ---- device arena: one hipMalloc per job, carved into 256-byte aligned pieces
+struct Arena {
+  size_t off = 0;
+  std::vector<std::pair<void **, size_t>> reqs;
+  template <class T>
+  void add(T **p, size_t n) {
+    reqs.push_back({(void **)p, (n * sizeof(T) + 255) / 256 * 256});
+    off += reqs.back().second;
+  }
+  gd_status commit(void **base) {
+    *base = nullptr;
+    if (off == 0) return GD_OK;
+    hipError_t e = hipMalloc(base, off);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    size_t o = 0;
+    for (auto &r : reqs) {
+      *r.first = (char *)*base + o;
+      o += r.second;
+    }
+    return GD_OK;
+  }
+};
+
+// Exact float threshold T on r^2 such that sqrtf(r2) <= pmf0  <=>  r2 <= T
+// (the reference tests dst - pmf0 <= 0 with dst = sqrtf(r2),
+// energy_prt_v1.C:88,100).  sqrtf is correctly rounded and monotone.
+static float sqrt_threshold(float pmf0) {
+  if (!(pmf0 >= 0.0f)) return -1.0f;  // never true for r2 >= 0
+  float x = (float)((double)pmf0 * (double)pmf0);
+  while (std::sqrt(x) > pmf0) x = std::nextafter(x, 0.0f);
+  for (;;) {
+    const float n = std::nextafter(x, INFINITY);
+    if (std::sqrt(n) <= pmf0 && std::isfinite(n))
+      x = n;
+    else
+      break;
+  }
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// Tiling: ligand chunk size lc (kernel template), protein tile ptile.  The
+// default aims for ~4 waves per SIMD over the whole launch; it depends only
+// on the job so a job's reduction order is the same alone or in a batch.
+bool tuned_tiling(int L, int P, int R, int &lc, int &ptile);  // gd_batch.hip
+
+void default_tiling(int L, int P, int R, int &lc, int &ptile) {
+  if (tuned_tiling(L, P, R, lc, ptile)) return;
+  lc = L <= 12 ? 4 : 8;
+  const int nL = (L + lc - 1) / lc;
+  const int nblk = (R + 63) / 64;
+  const int simds = 4 * (g_cu > 0 ? g_cu : 256);
+  const int target_waves = 4 * simds;
+  int nP = std::max(1, target_waves / std::max(1, nblk * nL));
+  nP = std::min(nP, std::max(1, P / 16));
+  ptile = (P + nP - 1) / nP;
+}
+
+static gd_status alloc_partials(gd_job *j, GdJob &g, int cap_rp) {
+  // partials depend on the tiling; they live in their own allocation
+  if (j->d_part) {
+    hipFree(j->d_part);
+    j->d_part = nullptr;
+  }
+  Arena a;
+  const size_t nS = (size_t)g.nL * g.nP;
+  a.add(&g.part_lin, (size_t)GD_NLIN * nS * cap_rp);
+  a.add(&g.part_hpc, (size_t)g.L * g.nP * cap_rp);
+  a.add(&g.part_mcs, (size_t)std::max(1, g.M) * g.nL * cap_rp);
+  return a.commit(&j->d_part);
+}
+
+gd_status apply_tiling(gd_job *j, int lc, int ptile) {
+  GdJob &g = j->h;
+  if (lc != 4 && lc != 8 && lc != 16) return GD_ERR_ARG;
+  ptile = std::max(1, std::min(ptile, g.P));
+  g.lc = lc;
+  g.nL = (g.L + lc - 1) / lc;
+  g.ptile = ptile;
+  g.nP = (g.P + ptile - 1) / ptile;
+  gd_status st = alloc_partials(j, g, g.Rp);
+  if (st != GD_OK) return st;
+  j->items.clear();
+  for (int rb = 0; rb < g.Rp; rb += 64)
+    for (int b = 0; b < g.nP; ++b)
+      for (int a = 0; a < g.nL; ++a) j->items.push_back({0, rb, a, b});
+  return GD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Pack a complex into the device layout.
+static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
+  const int L = d->lig_natom, P = d->prt_npoint, K = d->kde_npoint, M = d->mcs_nrow;
+  const int nl = d->n_lig, np = d->n_prt, nt = d->n_tmp;
+  if (L < 1 || P < 1 || K < 0 || M < 0 || nl < 1 || np < 1 || nt < 1) return GD_ERR_ARG;
+  if (!d->lig_x || !d->lig_y || !d->lig_z || !d->lig_t || !d->lig_c || !d->prt_x || !d->prt_y ||
+      !d->prt_z || !d->prt_t || !d->prt_c || !d->prt_cdc || !d->prt_seq3r || !d->prt_hpp ||
+      !d->prt_ele || !d->ene || !d->minus_beta || (K > 0 && (!d->kde_t || !d->kde_x)) ||
+      (M > 0 && (!d->mcs_x || !d->mcs_tcc)) || (d->psp_ncol > 0 && !d->psp))
+    return GD_ERR_ARG;
+  const long long R64 = (long long)nl * np * nt;
+  if (R64 > (1 << 26) || L > 4096 || P > (1 << 22)) return GD_ERR_SIZE;
+  for (int l = 0; l < L; ++l)
+    if (d->lig_t[l] < 0 || d->lig_t[l] >= GD_NLT) {
+      set_error("ligand atom type outside 0..23 (BADKDE)");
+      return GD_ERR_TYPE;
+    }
+  for (int p = 0; p < P; ++p)
+    if (d->prt_t[p] < 0 || d->prt_t[p] >= GD_NPT) {
+      set_error("protein point type outside 0..30");
+      return GD_ERR_TYPE;
+    }
+  const int R = (int)R64, Rp = (R + 63) / 64 * 64;
+  const gd_enepara *e = d->ene;
+  GdJob g;
+  std::memset(&g, 0, sizeof g);
+  g.L = L, g.P = P, g.K = K, g.M = M, g.n_lig = nl, g.n_prt = np, g.n_tmp = nt, g.R = R, g.Rp = Rp;
+
+  // distinct ligand types of the job -> parameter-table rows
+  std::vector<int> types;
+  std::vector<int> trow(GD_NLT, -1);
+  for (int l = 0; l < L; ++l)
+    if (trow[d->lig_t[l]] < 0) {
+      trow[d->lig_t[l]] = (int)types.size();
+      types.push_back(d->lig_t[l]);
+    }
+  const int nty = (int)types.size();
+  g.n_types = nty;
+  std::vector<float> ptab((size_t)nty * GD_PTAB_PT * GD_PTAB_W, 0.0f);
+  const float hdb_c = std::sqrt(0.5f * 1.44269504088896340736f);
+  for (int ti = 0; ti < nty; ++ti) {
+    const int lt = types[ti];
+    for (int pt = 0; pt < GD_NPT; ++pt) {
+      float *o = &ptab[((size_t)ti * GD_PTAB_PT + pt) * GD_PTAB_W];
+      o[0] = e->p1a[lt][pt];
+      o[1] = e->p2a[lt][pt];
+      o[2] = e->pmf0[lt][pt];
+      o[3] = e->pmf1[lt][pt];
+      const bool hb = e->hdb0[lt][pt] > 0.1f;
+      o[4] = hb ? e->hdb0[lt][pt] : 0.0f;
+      o[5] = hb ? e->hdb1[lt][pt] * hdb_c : 0.0f;
+      o[6] = hb ? e->hdb1[lt][pt] : 0.0f;
+      o[7] = sqrt_threshold(e->pmf0[lt][pt]);
+    }
+  }
+  std::vector<float> pspv((size_t)nty * P, 0.0f);
+  std::vector<float4> pattr(P);
+  for (int p = 0; p < P; ++p) {
+    const int c = d->prt_c[p];
+    int fl = d->prt_t[p] | (d->prt_cdc[p] == 1 ? GD_PF_CDC : 0) | (c == 2 ? GD_PF_CLS2 : 0);
+    float flf;
+    std::memcpy(&flf, &fl, 4);
+    pattr[p] = make_float4(d->prt_ele[p], d->prt_hpp[p], flf, 0.0f);
+    if (c == 2) {
+      const int res = d->prt_seq3r[p];
+      for (int ti = 0; ti < nty; ++ti)
+        pspv[(size_t)ti * P + p] =
+            (res >= 0 && res < d->psp_ncol) ? d->psp[(size_t)types[ti] * d->psp_ncol + res] : 0.0f;
+    }
+  }
+  // KDE bins (anchors sorted by type)
+  std::vector<float4> kde(std::max(1, K));
+  for (int k = 0; k < K; ++k) {
+    if (k > 0 && d->kde_t[k] < d->kde_t[k - 1]) {
+      set_error("KDE points must be sorted by type");
+      return GD_ERR_ARG;
+    }
+    kde[k] = make_float4(d->kde_x[k], d->kde_y[k], d->kde_z[k], 0.0f);
+  }
+  std::vector<GdAtom> atoms(L);
+  for (int l = 0; l < L; ++l) {
+    const int lt = d->lig_t[l];
+    GdAtom &a = atoms[l];
+    a.tidx = trow[lt];
+    a.q = d->lig_c[l];
+    int b = 0;
+    while (b < K && d->kde_t[b] < lt) ++b;
+    int en = b;
+    while (en < K && d->kde_t[en] == lt) ++en;
+    a.kde_b = b;
+    a.kde_e = en;
+    a.kde_inv_n = en > b ? 1.0f / (float)(en - b) : 0.0f;
+    a.hpl0 = e->hpl0[lt];
+    a.hpl1 = e->hpl1[lt];
+    a.hpl2 = e->hpl2[lt];
+  }
+  std::vector<float4> mcs((size_t)std::max(1, M) * L);
+  std::vector<float2> mcsrow(std::max(1, M));
+  for (int m = 0; m < M; ++m) {
+    int n = 0;
+    for (int l = 0; l < L; ++l) {
+      const size_t i = (size_t)m * L + l;
+      const bool v = d->mcs_x[i] != GD_MCS_INVALID;
+      n += v;
+      mcs[i] = v ? make_float4(d->mcs_x[i], d->mcs_y[i], d->mcs_z[i], 1.0f)
+                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    mcsrow[m] = make_float2(d->mcs_tcc[m], (float)n);
+  }
+  std::vector<float4> lig0((size_t)nl * L), pco((size_t)np * P);
+  for (int i = 0; i < nl; ++i)
+    for (int l = 0; l < L; ++l) {
+      const size_t q = (size_t)i * L + l;
+      lig0[q] = make_float4(d->lig_x[q], d->lig_y[q], d->lig_z[q], 0.0f);
+    }
+  for (int i = 0; i < np; ++i)
+    for (int p = 0; p < P; ++p) {
+      const size_t q = (size_t)i * P + p;
+      pco[q] = make_float4(d->prt_x[q], d->prt_y[q], d->prt_z[q], 0.0f);
+    }
+  // scalars
+  g.lj0 = e->lj0, g.lj1 = e->lj1, g.el0 = e->el0, g.el1 = e->el1, g.a1 = e->a1, g.b1 = e->b1;
+  g.inv_el1 = 1.0f / e->el1;
+  g.kde2l = e->kde2 * 1.44269504088896340736f;
+  g.kde3_inv = 1.0f / e->kde3;
+  g.inv_L = 1.0f / (float)L;
+  for (int i = 0; i < 3; ++i) g.center[i] = d->lig_center[i], g.pocket[i] = d->pocket_center[i];
+  for (int i = 0; i < 10; ++i) g.a_para[i] = e->a_para[i], g.b_para[i] = e->b_para[i], g.w[i] = e->w[i];
+
+  // device memory: constant tables + replica state
+  Arena a;
+  float4 *d_lig0, *d_pco, *d_pattr, *d_kde, *d_mcs;
+  GdAtom *d_atoms;
+  float *d_ptab, *d_pspv, *d_mb;
+  float2 *d_mcsrow;
+  a.add(&d_lig0, lig0.size());
+  a.add(&d_atoms, atoms.size());
+  a.add(&d_pco, pco.size());
+  a.add(&d_pattr, pattr.size());
+  a.add(&d_ptab, ptab.size());
+  a.add(&d_pspv, pspv.size());
+  a.add(&d_kde, kde.size());
+  a.add(&d_mcs, mcs.size());
+  a.add(&d_mcsrow, mcsrow.size());
+  a.add(&d_mb, (size_t)nt);
+  a.add(&g.mm, (size_t)6 * Rp);
+  a.add(&g.prop, (size_t)6 * Rp);
+  a.add(&g.en, (size_t)10 * Rp);
+  a.add(&g.ligi, (size_t)Rp);
+  a.add(&g.prti, (size_t)Rp);
+  a.add(&g.tmpi, (size_t)Rp);
+  a.add(&g.acc, (size_t)Rp);
+  a.add(&g.nacc, (size_t)Rp);
+  a.add(&g.nexc, (size_t)Rp);
+  a.add(&g.best, (size_t)Rp);
+  a.add(&g.own, (size_t)nl * np * nt);
+  a.add(&g.coords, (size_t)3 * L * Rp);
+  a.add(&g.rec_n, (size_t)Rp);
+  a.add(&g.ev_terms, (size_t)10 * Rp);
+  a.add(&g.ev_raw, (size_t)9 * Rp);
+  a.add(&g.wq, 64);
+  gd_status st = a.commit(&j->d_base);
+  if (st != GD_OK) return st;
+  j->bytes = a.off;
+  g.lig0 = d_lig0, g.atoms = d_atoms, g.pcoord = d_pco, g.pattr = d_pattr, g.ptab = d_ptab;
+  g.pspv = d_pspv, g.kde = d_kde, g.mcs = d_mcs, g.mcsrow = d_mcsrow, g.minus_beta = d_mb;
+  auto up = [&](void *dst, const void *src, size_t n) -> gd_status {
+    hipError_t e2 = hipMemcpy(dst, src, n, hipMemcpyHostToDevice);
+    return e2 == hipSuccess ? GD_OK : hip_fail(e2, "hipMemcpy H2D");
+  };
+  if ((st = up(d_lig0, lig0.data(), lig0.size() * sizeof(float4))) != GD_OK) return st;
+  if ((st = up(d_atoms, atoms.data(), atoms.size() * sizeof(GdAtom))) != GD_OK) return st;
+  if ((st = up(d_pco, pco.data(), pco.size() * sizeof(float4))) != GD_OK) return st;
+  if ((st = up(d_pattr, pattr.data(), pattr.size() * sizeof(float4))) != GD_OK) return st;
+  if ((st = up(d_ptab, ptab.data(), ptab.size() * sizeof(float))) != GD_OK) return st;
+  if ((st = up(d_pspv, pspv.data(), pspv.size() * sizeof(float))) != GD_OK) return st;
+  if ((st = up(d_kde, kde.data(), kde.size() * sizeof(float4))) != GD_OK) return st;
+  if ((st = up(d_mcs, mcs.data(), mcs.size() * sizeof(float4))) != GD_OK) return st;
+  if ((st = up(d_mcsrow, mcsrow.data(), mcsrow.size() * sizeof(float2))) != GD_OK) return st;
+  if ((st = up(d_mb, d->minus_beta, (size_t)nt * sizeof(float))) != GD_OK) return st;
+  j->h = g;
+  return GD_OK;
+}
+
+// Replica table of SetReplica (util.C:370-406): idx = n_tmp*n_lig*p + n_lig*t + l,
+// zero pose and energies; padding lanes get conformer 0 and never write back.
+// Device bytes gd_job_create allocates for a descriptor (tables, replica
+// state, partial sums under the default tiling, work lists).
+size_t footprint(const gd_job_desc *d) {
+  if (!d || d->lig_natom < 1 || d->prt_npoint < 1 || d->n_lig < 1 || d->n_prt < 1 || d->n_tmp < 1)
+    return 0;
+  const size_t L = d->lig_natom, P = d->prt_npoint, K = d->kde_npoint, M = d->mcs_nrow;
+  const size_t nl = d->n_lig, np = d->n_prt, nt = d->n_tmp;
+  const size_t R = nl * np * nt, Rp = (R + 63) / 64 * 64;
+  auto a = [](size_t bytes) { return (bytes + 255) / 256 * 256; };
+  std::vector<int> seen(GD_NLT, 0);
+  size_t nty = 0;
+  for (size_t l = 0; l < L; ++l)
+    if (d->lig_t[l] >= 0 && d->lig_t[l] < GD_NLT && !seen[d->lig_t[l]]++) ++nty;
+  size_t b = a(nl * L * 16) + a(L * sizeof(GdAtom)) + a(np * P * 16) + a(P * 16) +
+             a(nty * GD_PTAB_PT * GD_PTAB_W * 4) + a(nty * P * 4) + a(std::max<size_t>(1, K) * 16) +
+             a(std::max<size_t>(1, M) * L * 16) + a(std::max<size_t>(1, M) * 8) + a(nt * 4);
+  b += a(6 * Rp * 4) * 2 + a(10 * Rp * 4) + a(Rp * 4) * 6 + a(Rp * 4) + a(nl * np * nt * 4) +
+       a(3 * L * Rp * 4) + a(Rp * 4) + a(10 * Rp * 4) + a(9 * Rp * 4) + a(64 * 4);
+  int lc, ptile;
+  default_tiling((int)L, (int)P, (int)R, lc, ptile);
+  const size_t nL = (L + lc - 1) / lc, nP = (P + ptile - 1) / ptile;
+  b += a(GD_NLIN * nL * nP * Rp * 4) + a(L * nP * Rp * 4) + a(std::max<size_t>(1, M) * nL * Rp * 4);
+  b += (Rp / 64) * nL * nP * sizeof(GdItem) + (Rp / 64) * sizeof(GdBlock) + 512;
+  return b;
+}
+
+gd_status reset_replicas(gd_job *j) {
+  const GdJob &g = j->h;
+  const int Rp = g.Rp, nl = g.n_lig, np = g.n_prt, nt = g.n_tmp;
+  std::vector<int> ligi(Rp, 0), prti(Rp, 0), tmpi(Rp, 0), zero(Rp, 0), own((size_t)nl * np * nt);
+  for (int p = 0; p < np; ++p)
+    for (int t = 0; t < nt; ++t)
+      for (int l = 0; l < nl; ++l) {
+        const int r = nt * nl * p + nl * t + l;
+        ligi[r] = l, prti[r] = p, tmpi[r] = t;
+        own[((size_t)p * nl + l) * nt + t] = r;
+      }
+  std::vector<float> zf((size_t)10 * Rp, 0.0f), best(Rp, 1e30f);
+  hipStream_t s = j->stream;
+  hipError_t e;
+#define GD_UP(dst, src, n)                                                          \
+  if ((e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s)) != hipSuccess) \
+    return hip_fail(e, "hipMemcpyAsync");
+  GD_UP(g.ligi, ligi.data(), Rp * sizeof(int));
+  GD_UP(g.prti, prti.data(), Rp * sizeof(int));
+  GD_UP(g.tmpi, tmpi.data(), Rp * sizeof(int));
+  GD_UP(g.acc, zero.data(), Rp * sizeof(int));
+  GD_UP(g.nacc, zero.data(), Rp * sizeof(int));
+  GD_UP(g.nexc, zero.data(), Rp * sizeof(int));
+  GD_UP(g.rec_n, zero.data(), Rp * sizeof(int));
+  GD_UP(g.mm, zf.data(), (size_t)6 * Rp * sizeof(float));
+  GD_UP(g.prop, zf.data(), (size_t)6 * Rp * sizeof(float));
+  GD_UP(g.en, zf.data(), (size_t)10 * Rp * sizeof(float));
+  GD_UP(g.best, best.data(), Rp * sizeof(float));
+  GD_UP(g.own, own.data(), own.size() * sizeof(int));
+#undef GD_UP
+  e = hipStreamSynchronize(s);
+  return e == hipSuccess ? GD_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+gd_status sync_job_struct(gd_job *j) {
+  hipError_t e = hipMemcpyAsync(j->d_job, &j->h, sizeof(GdJob), hipMemcpyHostToDevice, j->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync job");
+  std::vector<GdBlock> blocks;
+  for (int rb = 0; rb < j->h.Rp; rb += 64) blocks.push_back({0, rb});
+  j->n_blocks = (int)blocks.size();
+  j->n_items = (int)j->items.size();
+  if (j->d_lists) hipFree(j->d_lists);
+  j->d_lists = nullptr;
+  const size_t bi = (size_t)j->n_items * sizeof(GdItem), bb = (size_t)j->n_blocks * sizeof(GdBlock);
+  e = hipMalloc(&j->d_lists, bi + bb + 256);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc lists");
+  j->d_items = (GdItem *)j->d_lists;
+  j->d_blocks = (GdBlock *)((char *)j->d_lists + (bi + 255) / 256 * 256);
+  if ((e = hipMemcpyAsync(j->d_items, j->items.data(), bi, hipMemcpyHostToDevice, j->stream)) != hipSuccess)
+    return hip_fail(e, "hipMemcpyAsync items");
+  if ((e = hipMemcpyAsync(j->d_blocks, blocks.data(), bb, hipMemcpyHostToDevice, j->stream)) != hipSuccess)
+    return hip_fail(e, "hipMemcpyAsync blocks");
+  e = hipStreamSynchronize(j->stream);
+  return e == hipSuccess ? GD_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+// ---- launches ---------------------------------------------------------------
+gd_status launch_pair(int lc, const GdJob *d_jobs, const GdItem *d_items, int n_items,
+                      hipStream_t s) {
+  if (n_items <= 0) return GD_OK;
+  const dim3 grid((n_items + 3) / 4), block(256);
+  switch (lc) {
+    case 4:
+      hipLaunchKernelGGL(gd_pair_kernel<4>, grid, block, 0, s, d_jobs, d_items, n_items);
+      break;
+    case 8:
+      hipLaunchKernelGGL(gd_pair_kernel<8>, grid, block, 0, s, d_jobs, d_items, n_items);
+      break;
+    case 16:
+      hipLaunchKernelGGL(gd_pair_kernel<16>, grid, block, 0, s, d_jobs, d_items, n_items);
+      break;
+    default:
+      return GD_ERR_ARG;
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GD_OK : hip_fail(e, "pair kernel launch");
+}
+
+gd_status launch_finish(const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks, int mode,
+                        int step, hipStream_t s) {
+  if (n_blocks <= 0) return GD_OK;
+  hipLaunchKernelGGL(gd_finish_kernel, dim3(n_blocks), dim3(256), 0, s, d_jobs, d_blocks, mode, step,
+                     (const int *)nullptr, (int *)nullptr);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GD_OK : hip_fail(e, "finish kernel launch");
+}
+
+gd_status launch_transform(const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks,
+                           hipStream_t s) {
+  if (n_blocks <= 0) return GD_OK;
+  hipLaunchKernelGGL(gd_transform_kernel, dim3(n_blocks), dim3(256), 0, s, d_jobs, d_blocks,
+                     (int *)nullptr);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GD_OK : hip_fail(e, "transform kernel launch");
+}
+
+gd_status launch_exchange(const GdJob *d_jobs, const GdExch *d_ex, int n_ex, hipStream_t s) {
+  if (n_ex <= 0) return GD_OK;
+  hipLaunchKernelGGL(gd_exchange_kernel, dim3(n_ex), dim3(256), 0, s, d_jobs, d_ex);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GD_OK : hip_fail(e, "exchange kernel launch");
+}
+
+void set_mc(GdJob &g, const gd_mc_param *mc) {
+  for (int i = 0; i < 6; ++i) g.move_scale[i] = mc->move_scale[i];
+  g.move_mode = mc->move_mode;
+  g.seed_lo = (uint32_t)mc->seed;
+  g.seed_hi = (uint32_t)(mc->seed >> 32);
+  g.job_id = mc->job_id;
+  g.record = mc->record;
+  g.steps_total = mc->steps_total;
+  g.step0 = 0;
+}
+
+gd_status read_out(gd_job *j, gd_replica_out *out) {
+  if (!out) return GD_OK;
+  const GdJob &g = j->h;
+  const int R = g.R, Rp = g.Rp;
+  hipError_t e;
+  std::vector<float> f((size_t)10 * Rp);
+  std::vector<int> iv(Rp);
+  auto down = [&](void *dst, const void *src, size_t n) -> gd_status {
+    e = hipMemcpy(dst, src, n, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? GD_OK : hip_fail(e, "hipMemcpy D2H");
+  };
+  gd_status st;
+  if (out->movematrix) {
+    if ((st = down(f.data(), g.mm, (size_t)6 * Rp * sizeof(float))) != GD_OK) return st;
+    for (int r = 0; r < R; ++r)
+      for (int i = 0; i < 6; ++i) out->movematrix[(size_t)r * 6 + i] = f[(size_t)i * Rp + r];
+  }
+  if (out->energy) {
+    if ((st = down(f.data(), g.en, (size_t)10 * Rp * sizeof(float))) != GD_OK) return st;
+    for (int r = 0; r < R; ++r)
+      for (int i = 0; i < 10; ++i) out->energy[(size_t)r * 10 + i] = f[(size_t)i * Rp + r];
+  }
+  if (out->best_energy) {
+    if ((st = down(f.data(), g.best, (size_t)Rp * sizeof(float))) != GD_OK) return st;
+    std::memcpy(out->best_energy, f.data(), (size_t)R * sizeof(float));
+  }
+  struct {
+    int *dst;
+    int *src;
+  } ints[] = {{out->idx_tmp, g.tmpi}, {out->n_accept, g.nacc}, {out->n_exchange, g.nexc},
+              {out->is_accept, g.acc}};
+  for (auto &q : ints)
+    if (q.dst) {
+      if ((st = down(iv.data(), q.src, (size_t)Rp * sizeof(int))) != GD_OK) return st;
+      std::memcpy(q.dst, iv.data(), (size_t)R * sizeof(int));
+    }
+  return GD_OK;
+}
+
+}  // namespace gdr
+
+using namespace gdr;
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+gd_status gd_init(int dev) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    set_error("no HIP device visible");
+    return GD_ERR_NODEV;
+  }
+  if (dev < 0 || dev >= n) return GD_ERR_ARG;
+  hipDeviceProp_t prop;
+  if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error(std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+    return GD_ERR_NODEV;
+  }
+  if ((e = hipSetDevice(dev)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+  g_device = dev;
+  g_cu = prop.multiProcessorCount;
+  return GD_OK;
+}
+
+gd_status gd_device_info(char *name, int name_len, int *n_cu_out, int *clock_khz, size_t *hbm) {
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, g_device);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  if (name && name_len > 0) std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  if (n_cu_out) *n_cu_out = prop.multiProcessorCount;
+  if (clock_khz) *clock_khz = prop.clockRate;
+  if (hbm) *hbm = prop.totalGlobalMem;
+  return GD_OK;
+}
+
+const char *gd_last_error(void) { return last_error(); }
+const char *gd_version(void) { return "geauxdock_amd 0.1 (gfx950)"; }
+
+void gd_philox(const uint32_t ctr[4], const uint32_t key[2], int rounds, uint32_t out[4]) {
+  const gd_u32x4 r = gd_philox4x32(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], rounds);
+  for (int i = 0; i < 4; ++i) out[i] = r.v[i];
+}
+
+gd_status gd_job_create(const gd_job_desc *desc, gd_job **out) {
+  if (!desc || !out) return GD_ERR_ARG;
+  *out = nullptr;
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  gd_job *j = new gd_job();
+  hipError_t e = hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete j;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  if ((e = hipEventCreate(&j->ev0)) != hipSuccess || (e = hipEventCreate(&j->ev1)) != hipSuccess) {
+    gd_job_destroy(j);
+    return hip_fail(e, "hipEventCreate");
+  }
+  st = pack_job(desc, j);
+  if (st == GD_OK) {
+    e = hipMalloc(&j->d_job, sizeof(GdJob));
+    if (e != hipSuccess) st = hip_fail(e, "hipMalloc job");
+  }
+  if (st == GD_OK) {
+    int lc, ptile;
+    default_tiling(j->h.L, j->h.P, j->h.R, lc, ptile);
+    st = apply_tiling(j, lc, ptile);
+  }
+  if (st == GD_OK) st = sync_job_struct(j);
+  if (st != GD_OK) {
+    gd_job_destroy(j);
+    return st;
+  }
+  *out = j;
+  return GD_OK;
+}
+
+gd_status gd_job_destroy(gd_job *j) {
+  if (!j) return GD_OK;
+  if (j->d_base) hipFree(j->d_base);
+  if (j->d_part) hipFree(j->d_part);
+  if (j->d_job) hipFree(j->d_job);
+  if (j->d_lists) hipFree(j->d_lists);
+  if (j->d_rec) hipFree(j->d_rec);
+  if (j->ev0) hipEventDestroy(j->ev0);
+  if (j->ev1) hipEventDestroy(j->ev1);
+  if (j->stream) hipStreamDestroy(j->stream);
+  delete j;
+  return GD_OK;
+}
+
+gd_status gd_job_sizes(const gd_job *j, int *n_rep, int *L, int *P, int *K, int *M) {
+  if (!j) return GD_ERR_ARG;
+  if (n_rep) *n_rep = j->h.R;
+  if (L) *L = j->h.L;
+  if (P) *P = j->h.P;
+  if (K) *K = j->h.K;
+  if (M) *M = j->h.M;
+  return GD_OK;
+}
+
+size_t gd_job_footprint(const gd_job_desc *d) { return footprint(d); }
+
+size_t gd_job_device_bytes(const gd_job *j) {
+  if (!j) return 0;
+  const GdJob &g = j->h;
+  const size_t part = ((size_t)GD_NLIN * g.nL * g.nP + (size_t)g.L * g.nP + (size_t)g.M * g.nL) *
+                      g.Rp * sizeof(float);
+  return j->bytes + part;
+}
+
+gd_status gd_tiling_get(const gd_job *j, gd_tiling *t) {
+  if (!j || !t) return GD_ERR_ARG;
+  t->atoms_per_lane = j->h.lc;
+  t->tile_points = j->h.ptile;
+  t->waves_per_cu = 0;
+  return GD_OK;
+}
+
+gd_status gd_tiling_set(gd_job *j, const gd_tiling *t) {
+  if (!j || !t) return GD_ERR_ARG;
+  gd_status st = apply_tiling(j, t->atoms_per_lane, t->tile_points);
+  if (st != GD_OK) return st;
+  return sync_job_struct(j);
+}
+
+// Energies of arbitrary poses through the same pair/finish kernels: the poses
+// are loaded as a temporary replica set (chunks of at most the job's padded
+// replica count), transformed, evaluated in EVAL mode, and the job's own
+// replica state is restored afterwards.
+gd_status gd_energy_eval(gd_job *j, int n, const int *conf, const float *pose, float *terms,
+                         float *raw, float *coords) {
+  if (!j || n < 0 || (n > 0 && (!conf || !pose || !terms))) return GD_ERR_ARG;
+  if (n == 0) return GD_OK;
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  GdJob &g = j->h;
+  for (int i = 0; i < n; ++i)
+    if (conf[2 * i] < 0 || conf[2 * i] >= g.n_lig || conf[2 * i + 1] < 0 || conf[2 * i + 1] >= g.n_prt)
+      return GD_ERR_ARG;
+  const int Rp = g.Rp, L = g.L;
+  hipStream_t s = j->stream;
+  // save the replica conformer maps, restore at the end
+  std::vector<int> ligi_save(Rp), prti_save(Rp);
+  std::vector<float> prop_save((size_t)6 * Rp);
+  hipError_t e;
+  if ((e = hipMemcpy(ligi_save.data(), g.ligi, Rp * sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess ||
+      (e = hipMemcpy(prti_save.data(), g.prti, Rp * sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess ||
+      (e = hipMemcpy(prop_save.data(), g.prop, (size_t)6 * Rp * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)
+    return hip_fail(e, "hipMemcpy save");
+  std::vector<int> li(Rp), pi(Rp);
+  std::vector<float> pp((size_t)6 * Rp), ot((size_t)10 * Rp), orw((size_t)9 * Rp),
+      oc((size_t)3 * L * Rp);
+  const int saveR = g.R;
+  for (int base = 0; base < n && st == GD_OK; base += Rp) {
+    const int cnt = std::min(Rp, n - base);
+    std::fill(li.begin(), li.end(), 0);
+    std::fill(pi.begin(), pi.end(), 0);
+    std::fill(pp.begin(), pp.end(), 0.0f);
+    for (int i = 0; i < cnt; ++i) {
+      li[i] = conf[2 * (base + i)];
+      pi[i] = conf[2 * (base + i) + 1];
+      for (int k = 0; k < 6; ++k) pp[(size_t)k * Rp + i] = pose[6 * (size_t)(base + i) + k];
+    }
+    g.R = cnt;
+    if ((e = hipMemcpyAsync(j->d_job, &g, sizeof(GdJob), hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(g.ligi, li.data(), Rp * sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(g.prti, pi.data(), Rp * sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(g.prop, pp.data(), (size_t)6 * Rp * sizeof(float), hipMemcpyHostToDevice, s)) != hipSuccess) {
+      st = hip_fail(e, "hipMemcpyAsync eval");
+      break;
+    }
+    if ((st = launch_transform(j->d_job, j->d_blocks, j->n_blocks, s)) != GD_OK) break;
+    if ((st = launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s)) != GD_OK) break;
+    if ((st = launch_finish(j->d_job, j->d_blocks, j->n_blocks, GD_MODE_EVAL, 0, s)) != GD_OK) break;
+    if ((e = hipMemcpyAsync(ot.data(), g.ev_terms, (size_t)10 * Rp * sizeof(float), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(orw.data(), g.ev_raw, (size_t)9 * Rp * sizeof(float), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (coords && (e = hipMemcpyAsync(oc.data(), g.coords, (size_t)3 * L * Rp * sizeof(float), hipMemcpyDeviceToHost, s)) != hipSuccess) ||
+        (e = hipStreamSynchronize(s)) != hipSuccess) {
+      st = hip_fail(e, "eval readback");
+      break;
+    }
+    for (int i = 0; i < cnt; ++i) {
+      for (int k = 0; k < 10; ++k) terms[(size_t)(base + i) * 10 + k] = ot[(size_t)i * 10 + k];
+      if (raw)
+        for (int k = 0; k < 9; ++k) raw[(size_t)(base + i) * 9 + k] = orw[(size_t)i * 9 + k];
+      if (coords)
+        for (int l = 0; l < L; ++l)
+          for (int d = 0; d < 3; ++d)
+            coords[((size_t)(base + i) * L + l) * 3 + d] = oc[(size_t)(3 * l + d) * Rp + i];
+    }
+  }
+  g.R = saveR;
+  hipMemcpyAsync(g.ligi, ligi_save.data(), Rp * sizeof(int), hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(g.prti, prti_save.data(), Rp * sizeof(int), hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(g.prop, prop_save.data(), (size_t)6 * Rp * sizeof(float), hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(j->d_job, &g, sizeof(GdJob), hipMemcpyHostToDevice, s);
+  e = hipStreamSynchronize(s);
+  if (st == GD_OK && e != hipSuccess) st = hip_fail(e, "eval restore");
+  return st;
+}
+
+// Plain one-job docking (the reference Dock shape, backend_gpu.cu:64-133):
+// initial-energy pass, then steps_total steps in dump segments; temperature
+// exchange every steps_per_exchange steps; with mc->record the accept records
+// of the last segment stay on the device for gd_dock_records.
+gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stats *stats) {
+  if (!j || !mc || mc->steps_total < 0 || mc->steps_per_exchange < 0) return GD_ERR_ARG;
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  GdJob &g = j->h;
+  hipStream_t s = j->stream;
+  set_mc(g, mc);
+  const int spd = mc->steps_per_dump > 0 ? mc->steps_per_dump : std::max(1, mc->steps_total);
+  // record ring: one segment of steps_per_dump entries per replica
+  if (mc->record) {
+    const size_t need = (size_t)g.Rp * spd * GD_REC_W * sizeof(float);
+    if (need > j->rec_bytes) {
+      if (j->d_rec) hipFree(j->d_rec);
+      j->d_rec = nullptr;
+      j->rec_bytes = 0;
+      hipError_t e = hipMalloc(&j->d_rec, need);
+      if (e != hipSuccess) return hip_fail(e, "hipMalloc records");
+      j->rec_bytes = need;
+    }
+    g.rec = (float *)j->d_rec;
+    g.rec_cap = spd;
+  } else {
+    g.rec = nullptr;
+    g.rec_cap = 0;
+  }
+  if ((st = reset_replicas(j)) != GD_OK) return st;
+  hipError_t e = hipMemcpyAsync(j->d_job, &g, sizeof(GdJob), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync job");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  const int spe = mc->steps_per_exchange;
+  const bool exch = spe > 0 && g.n_tmp > 1;
+  const int n_rounds = exch ? mc->steps_total / spe : 0;
+  GdExch *d_ex = nullptr;
+  if (n_rounds > 0) {
+    std::vector<GdExch> ex(n_rounds);
+    for (int k = 0; k < n_rounds; ++k) ex[k] = {0, k};
+    e = hipMalloc(&d_ex, sizeof(GdExch) * n_rounds);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc exch");
+    e = hipMemcpy(d_ex, ex.data(), sizeof(GdExch) * n_rounds, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(d_ex);
+      return hip_fail(e, "hipMemcpy exch");
+    }
+  }
+  KTimer tp;
+  const bool timing = stats != nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
+  // initial energies (MonteCarlo_d with s2max = 1)
+  if (st == GD_OK) st = launch_transform(j->d_job, j->d_blocks, j->n_blocks, s);
+  if (st == GD_OK) st = launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s);
+  if (st == GD_OK) st = launch_finish(j->d_job, j->d_blocks, j->n_blocks, GD_MODE_INIT, 0, s);
+  for (int s1 = 0; s1 < mc->steps_total && st == GD_OK; s1 += spd) {
+    if (mc->record) {
+      e = hipMemsetAsync(g.rec_n, 0, (size_t)g.Rp * sizeof(int), s);
+      if (e != hipSuccess) {
+        st = hip_fail(e, "hipMemsetAsync");
+        break;
+      }
+    }
+    const int s_end = std::min(mc->steps_total, s1 + spd);
+    for (int step = s1; step < s_end; ++step) {
+      if (timing && (st = tp.begin(s)) != GD_OK) break;
+      if ((st = launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s)) != GD_OK) break;
+      if (timing && (st = tp.end(s)) != GD_OK) break;
+      if ((st = launch_finish(j->d_job, j->d_blocks, j->n_blocks, GD_MODE_STEP, step, s)) != GD_OK) break;
+      if (d_ex && (step + 1) % spe == 0 && (step + 1) / spe <= n_rounds)
+        if ((st = launch_exchange(j->d_job, d_ex + ((step + 1) / spe - 1), 1, s)) != GD_OK) break;
+    }
+    e = hipStreamSynchronize(s);  // segment boundary (the reference's blocking D2H)
+    if (st == GD_OK && e != hipSuccess) st = hip_fail(e, "MC segment");
+    if (st == GD_OK && timing) st = tp.collect();
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  if (d_ex) hipFree(d_ex);
+  if (st != GD_OK) return st;
+  if (stats) {
+    stats->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    stats->ms_pair_kernel = tp.ms;
+    stats->ms_finish_kernel = 0.0;
+    stats->n_pair_launch = tp.n;
+    stats->pair_evals = (double)g.R * g.L * g.P * (mc->steps_total + 1);
+    stats->replica_steps = (double)g.R * mc->steps_total;
+  }
+  return read_out(j, out);
+}
+
+gd_status gd_dock_records(gd_job *j, int cap, float *rec, int *count) {
+  if (!j || cap < 0 || !count || (cap > 0 && !rec)) return GD_ERR_ARG;
+  const GdJob &g = j->h;
+  if (!g.rec) return GD_ERR_STATE;
+  const int R = g.R, Rp = g.Rp, rc = g.rec_cap;
+  std::vector<int> n(Rp);
+  hipError_t e = hipMemcpy(n.data(), g.rec_n, (size_t)Rp * sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy rec_n");
+  std::vector<float> buf((size_t)Rp * rc * GD_REC_W);
+  if (!buf.empty()) {
+    e = hipMemcpy(buf.data(), g.rec, buf.size() * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy records");
+  }
+  for (int r = 0; r < R; ++r) {
+    const int m = std::min(n[r], cap);
+    count[r] = n[r];
+    for (int k = 0; k < m; ++k)
+      for (int w = 0; w < GD_REC_W; ++w)
+        rec[((size_t)r * cap + k) * GD_REC_W + w] = buf[((size_t)r * rc + k) * GD_REC_W + w];
+  }
+  return GD_OK;
+}
+
+}  // extern "C"

@@ -210,17 +210,26 @@ typedef struct {
 
 gd_status gd_server_create(const gd_server_config *cfg, gd_server **out);
 gd_status gd_server_destroy(gd_server *s);
-/* Takes ownership of `job`; out (optional) receives the final replica state
- * when the job retires (arrays must stay valid until then). */
-gd_status gd_server_submit(gd_server *s, gd_job *job, const gd_mc_param *mc,
+/* Device bytes a complex will occupy once admitted (host-side estimate that
+ * matches what gd_job_create allocates with the default tiling). */
+size_t gd_job_footprint(const gd_job_desc *desc);
+/* Queue a docking request.  The descriptor's arrays must stay valid until the
+ * request is admitted (its results are ready or it was cancelled); out
+ * (optional) receives the final replica state when the job retires.  The
+ * server creates the device job at admission, when the job's footprint fits
+ * the HBM budget and an active slot is free. */
+gd_status gd_server_submit(gd_server *s, const gd_job_desc *desc, const gd_mc_param *mc,
                            gd_replica_out *out, int64_t *ticket);
-/* One sweep: admit what fits, advance every active job, retire finished
- * ones.  *n_done receives how many results are now ready. */
+/* One sweep: admit what fits, advance every active job by up to sweep_steps
+ * MC steps in shared launches, retire finished jobs.  *n_active / *n_done
+ * receive the active count and the number of results ready to pop. */
 gd_status gd_server_step(gd_server *s, int *n_active, int *n_done);
 gd_status gd_server_cancel(gd_server *s, int64_t ticket);
 /* Pop up to `cap` finished results. */
 gd_status gd_server_results(gd_server *s, gd_job_result *res, int cap, int *n);
 double gd_server_clock(gd_server *s);
+/* Bytes of the budget in use and requests still waiting for admission. */
+gd_status gd_server_usage(gd_server *s, size_t *bytes_used, int *n_pending);
 
 /* ---- loaders (reference frontend, re-implemented) ----------------------- */
 typedef struct gd_host_job gd_host_job;   /* host-side complex built by the loaders */

@@ -136,13 +136,24 @@ inline float t_log(float x) { return logf(x); }
 template <>
 inline double t_log(double x) { return log(x); }
 
-// Ligand transform of kernel_cpp_montecarlo_soa.C:256-293 (float).
-void transform(const gd_job_desc *d, int lig, const float m[6], float *x, float *y, float *z) {
+// Ligand transform of kernel_cpp_montecarlo_soa.C:256-293 (float in the
+// FP32 modes, double in the FP64 twin).
+template <typename T>
+void transform(const gd_job_desc *d, int lig, const float mf[6], T *x, T *y, T *z) {
   const int L = d->lig_natom;
-  const float s1 = sinf(m[3]), c1 = cosf(m[3]);
-  const float s2 = sinf(m[4]), c2 = cosf(m[4]);
-  const float s3 = sinf(m[5]), c3 = cosf(m[5]);
-  float r[3][3];
+  T m[6];
+  for (int i = 0; i < 6; ++i) m[i] = (T)mf[i];
+  T s1, c1, s2, c2, s3, c3;
+  if (sizeof(T) == 4) {
+    s1 = sinf((float)m[3]), c1 = cosf((float)m[3]);
+    s2 = sinf((float)m[4]), c2 = cosf((float)m[4]);
+    s3 = sinf((float)m[5]), c3 = cosf((float)m[5]);
+  } else {
+    s1 = sin((double)m[3]), c1 = cos((double)m[3]);
+    s2 = sin((double)m[4]), c2 = cos((double)m[4]);
+    s3 = sin((double)m[5]), c3 = cos((double)m[5]);
+  }
+  T r[3][3];
   r[0][0] = c1 * c2;
   r[0][1] = c1 * s2 * s3 - c3 * s1;
   r[0][2] = s1 * s3 + c1 * c3 * s2;
@@ -155,17 +166,18 @@ void transform(const gd_job_desc *d, int lig, const float m[6], float *x, float 
   const float *cx = d->lig_x + (size_t)lig * L, *cy = d->lig_y + (size_t)lig * L,
               *cz = d->lig_z + (size_t)lig * L;
   for (int l = 0; l < L; ++l) {
-    x[l] = r[0][0] * cx[l] + r[0][1] * cy[l] + r[0][2] * cz[l] + m[0] + d->lig_center[0];
-    y[l] = r[1][0] * cx[l] + r[1][1] * cy[l] + r[1][2] * cz[l] + m[1] + d->lig_center[1];
-    z[l] = r[2][0] * cx[l] + r[2][1] * cy[l] + r[2][2] * cz[l] + m[2] + d->lig_center[2];
+    const T X = cx[l], Y = cy[l], Z = cz[l];
+    x[l] = r[0][0] * X + r[0][1] * Y + r[0][2] * Z + m[0] + (T)d->lig_center[0];
+    y[l] = r[1][0] * X + r[1][1] * Y + r[1][2] * Z + m[1] + (T)d->lig_center[1];
+    z[l] = r[2][0] * X + r[2][1] * Y + r[2][2] * Z + m[2] + (T)d->lig_center[2];
   }
 }
 
 // Full energy at given transformed coordinates (x,y,z in type-sorted order).
 // literal = reference expression order; otherwise the shared fma rounding.
 template <typename T>
-void energy(const Job &J, int prt, const float m[6], const float *xf, const float *yf,
-            const float *zf, bool literal, Out<T> &o) {
+void energy(const Job &J, int prt, const float m[6], const T *xf, const T *yf, const T *zf,
+            bool literal, Out<T> &o) {
   const gd_job_desc *d = J.d;
   const gd_enepara *e = d->ene;
   const int L = J.L, P = J.P, K = J.K, M = J.M;
@@ -320,9 +332,53 @@ void energy(const Job &J, int prt, const float m[6], const float *xf, const floa
 // Metropolis test of :382-385; the forced initial accept draws u = 0.
 inline bool accept(float u, float de, float mb) { return u < expf(de * mb); }
 
+// Temperature exchange of one ladder in round `round`: pairs (t, t+1) for
+// t = round mod 2, t+2, ...; swap temperature labels iff
+// u < exp((mb[t+1] - mb[t]) * (E_i - E_j)), u from counter
+// (round, ladder*n_tmp + t, job, 2).
+void exchange_ladder(const gd_mc_param *mc, const float *mb, int nt, uint32_t ladder, int round,
+                     int *own, const float *en, int *tmp, int *nexc) {
+  for (int t = round & 1; t + 1 < nt; t += 2) {
+    const int i = own[t], j = own[t + 1];
+    const P4 w = philox((uint32_t)round, ladder * (uint32_t)nt + (uint32_t)t, mc->job_id, 2u,
+                        (uint32_t)mc->seed, (uint32_t)(mc->seed >> 32), 10);
+    const float u = u01(w.v[0]);
+    const float x = (mb[t + 1] - mb[t]) * (en[(size_t)i * 10 + 9] - en[(size_t)j * 10 + 9]);
+    if (u < expf(x)) {
+      own[t] = j, own[t + 1] = i;
+      tmp[i] = t + 1, tmp[j] = t;
+      ++nexc[i], ++nexc[j];
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+// Exchange rule on a frozen ladder with the given total energies: runs
+// `rounds` exchange rounds, returns swaps per attempted pair and the final
+// temperature of each replica (test of the acceptance rule in isolation).
+double or_exchange_toy(int nt, const float *energy, const float *mb, int rounds, uint64_t seed,
+                       int *final_tmp) {
+  gd_mc_param mc;
+  memset(&mc, 0, sizeof mc);
+  mc.seed = seed;
+  std::vector<float> en((size_t)nt * 10, 0.0f);
+  std::vector<int> own(nt), tmp(nt), nexc(nt, 0);
+  for (int t = 0; t < nt; ++t) en[(size_t)t * 10 + 9] = energy[t], own[t] = t, tmp[t] = t;
+  long attempts = 0;
+  for (int k = 0; k < rounds; ++k) {
+    for (int t = k & 1; t + 1 < nt; t += 2) ++attempts;
+    exchange_ladder(&mc, mb, nt, 0u, k, own.data(), en.data(), tmp.data(), nexc.data());
+  }
+  long swaps = 0;
+  for (int t = 0; t < nt; ++t) swaps += nexc[t];
+  if (final_tmp)
+    for (int t = 0; t < nt; ++t) final_tmp[t] = tmp[t];
+  return attempts ? (double)swaps / 2.0 / (double)attempts : 0.0;
+}
+
 
 // Philox4x32-R for the known-answer tests.
 void or_philox(const uint32_t ctr[4], const uint32_t key[2], int rounds, uint32_t out[4]) {
@@ -346,24 +402,24 @@ int or_energy(const gd_job_desc *d, int n, const int *conf, const float *pose,
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
   for (int i = 0; i < n; ++i) {
-    std::vector<float> x(L), y(L), z(L);
     const float *m = pose + 6 * (size_t)i;
-    if (coords_in) {
-      for (int l = 0; l < L; ++l) {
-        x[l] = coords_in[((size_t)i * L + l) * 3 + 0];
-        y[l] = coords_in[((size_t)i * L + l) * 3 + 1];
-        z[l] = coords_in[((size_t)i * L + l) * 3 + 2];
-      }
-    } else {
-      transform(d, conf[2 * i], m, x.data(), y.data(), z.data());
-    }
-    if (coords_out)
-      for (int l = 0; l < L; ++l) {
-        coords_out[((size_t)i * L + l) * 3 + 0] = x[l];
-        coords_out[((size_t)i * L + l) * 3 + 1] = y[l];
-        coords_out[((size_t)i * L + l) * 3 + 2] = z[l];
-      }
     if (mode == 2) {
+      std::vector<double> x(L), y(L), z(L);
+      if (coords_in) {
+        for (int l = 0; l < L; ++l) {
+          x[l] = coords_in[((size_t)i * L + l) * 3 + 0];
+          y[l] = coords_in[((size_t)i * L + l) * 3 + 1];
+          z[l] = coords_in[((size_t)i * L + l) * 3 + 2];
+        }
+      } else {
+        transform<double>(d, conf[2 * i], m, x.data(), y.data(), z.data());
+      }
+      if (coords_out)
+        for (int l = 0; l < L; ++l) {
+          coords_out[((size_t)i * L + l) * 3 + 0] = (float)x[l];
+          coords_out[((size_t)i * L + l) * 3 + 1] = (float)y[l];
+          coords_out[((size_t)i * L + l) * 3 + 2] = (float)z[l];
+        }
       Out<double> o;
       energy<double>(J, conf[2 * i + 1], m, x.data(), y.data(), z.data(), false, o);
       for (int k = 0; k < 10; ++k)
@@ -373,6 +429,22 @@ int or_energy(const gd_job_desc *d, int n, const int *conf, const float *pose,
         if (scale) scale[(size_t)i * 9 + k] = o.scale[k];
       }
     } else {
+      std::vector<float> x(L), y(L), z(L);
+      if (coords_in) {
+        for (int l = 0; l < L; ++l) {
+          x[l] = coords_in[((size_t)i * L + l) * 3 + 0];
+          y[l] = coords_in[((size_t)i * L + l) * 3 + 1];
+          z[l] = coords_in[((size_t)i * L + l) * 3 + 2];
+        }
+      } else {
+        transform<float>(d, conf[2 * i], m, x.data(), y.data(), z.data());
+      }
+      if (coords_out)
+        for (int l = 0; l < L; ++l) {
+          coords_out[((size_t)i * L + l) * 3 + 0] = x[l];
+          coords_out[((size_t)i * L + l) * 3 + 1] = y[l];
+          coords_out[((size_t)i * L + l) * 3 + 2] = z[l];
+        }
       Out<float> o;
       energy<float>(J, conf[2 * i + 1], m, x.data(), y.data(), z.data(), mode == 1, o);
       for (int k = 0; k < 10; ++k)
@@ -436,7 +508,7 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
           for (int i = 0; i < 6; ++i) m[i] = inc[i] + mm[(size_t)r * 6 + i];
           u = u01(dr.w[6]);
         }
-        transform(d, lig[r], m, x.data(), y.data(), z.data());
+        transform<float>(d, lig[r], m, x.data(), y.data(), z.data());
         energy<float>(J, prt[r], m, x.data(), y.data(), z.data(), false, o);
         const float de = o.term[9] - en[(size_t)r * 10 + 9];
         const bool a = accept(u, de, d->minus_beta[tmp[r]]);
@@ -456,21 +528,11 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
       for (int p = 0; p < np; ++p)
         for (int l = 0; l < nl; ++l) {
           int *own = &owner[((size_t)p * nl + l) * nt];
-          for (int t = round & 1; t + 1 < nt; t += 2) {
-            const int i = own[t], j = own[t + 1];
-            if (i < rep_lo || i >= rep_hi || j < rep_lo || j >= rep_hi) continue;
-            const uint32_t pair = (uint32_t)(((size_t)p * nl + l) * nt + t);
-            const P4 w = philox((uint32_t)round, pair, mc->job_id, 2u, (uint32_t)mc->seed,
-                                (uint32_t)(mc->seed >> 32), 10);
-            const float u = u01(w.v[0]);
-            const float x = (d->minus_beta[t + 1] - d->minus_beta[t]) *
-                            (en[(size_t)i * 10 + 9] - en[(size_t)j * 10 + 9]);
-            if (u < expf(x)) {
-              own[t] = j, own[t + 1] = i;
-              tmp[i] = t + 1, tmp[j] = t;
-              ++nexc[i], ++nexc[j];
-            }
-          }
+          bool inside = true;
+          for (int t = 0; t < nt; ++t) inside = inside && own[t] >= rep_lo && own[t] < rep_hi;
+          if (!inside) continue;
+          exchange_ladder(mc, d->minus_beta, nt, (uint32_t)((size_t)p * nl + l), round, own,
+                          en.data(), tmp.data(), nexc.data());
         }
     }
   }

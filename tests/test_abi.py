@@ -1,0 +1,114 @@
+"""C-ABI surface and host-side logic of libgeauxdock_amd.so (CPU only: no
+kernel is launched here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import geauxdock_amd as g
+from geauxdock_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
+
+
+def _declared(path):
+    src = open(path).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"//[^\n]*", "", src)
+    names = set(re.findall(r"\b(gd_[a-z0-9_]+)\s*\(", src))
+    return names
+
+
+def test_library_loads():
+    L = g.lib()
+    assert L.gd_version().startswith(b"geauxdock_amd")
+
+
+@pytest.mark.parametrize("header", HEADERS, ids=os.path.basename)
+def test_every_declared_symbol_is_exported(header):
+    L = C.CDLL(g.LIB_PATH)
+    missing = [n for n in sorted(_declared(header)) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_dropin_symbols_exported():
+    """The reference's own entry points (C++ linkage), for main_gpu/geauxdock.C."""
+    out = os.popen("nm -D --defined-only %s" % g.LIB_PATH).read()
+    for mangled in ["_Z9SetDevicev", "_Z4DockP7ComplexP6RecordPS0_PS2_PP5ParaTPP12gd_rng_state",
+                    "_Z11DeviceAllocPP7ComplexPP6RecordPP12gd_rng_state",
+                    "_Z10DeviceFreePP7ComplexPP6RecordPP12gd_rng_state",
+                    "_Z8SetParaTP7ComplexPP5ParaT", "_Z7CopyH2DP7ComplexPS0_PP5ParaT",
+                    "_Z10InitCurandPP12gd_rng_state"]:
+        assert mangled in out, mangled
+
+
+def test_python_prototypes_match_header():
+    declared = set()
+    for h in HEADERS:
+        if h.endswith("geauxdock_amd.h"):
+            declared |= _declared(h)
+    assert declared == set(abi.PROTOTYPES), declared ^ set(abi.PROTOTYPES)
+
+
+def test_missing_file_is_an_error_not_an_exit(ene, tmp_path):
+    h = C.c_void_p()
+    st = g.lib().gd_load_complex(b"/nonexistent.sdf", b"/nonexistent.ff", b"/nonexistent.pdb",
+                                 b"x", C.byref(ene), 1, 0.044, 0.036, C.byref(h))
+    assert st == abi.GD_ERR_IO
+
+
+def test_unknown_ligand_type_rejected(ene, tmp_path):
+    """An unknown SYBYL name (reference BADKDE = 50, an out-of-bounds table
+    index there) rejects the job."""
+    src = open(os.path.join(ROOT, "tests", "golden", "data", "1a07C1.sdf")).read()
+    bad = src.replace("C.3 C.3 C.3 O.co2", "C.3 Xx C.3 O.co2", 1)
+    p = tmp_path / "bad.sdf"
+    p.write_text(bad)
+    h = C.c_void_p()
+    d = os.path.join(ROOT, "tests", "golden", "data")
+    st = g.lib().gd_load_complex(str(p).encode(), os.path.join(d, "1a07C1-0.8.ff").encode(),
+                                 os.path.join(d, "1a07C-1.pdb").encode(), b"1a07C1", C.byref(ene),
+                                 1, 0.044, 0.036, C.byref(h))
+    assert st == abi.GD_ERR_TYPE
+
+
+def test_synthetic_sizes_and_determinism(ene):
+    a = g.synth_complex(1200, 24, 5, 5, 1, seed=1, ene=ene)
+    b = g.synth_complex(1200, 24, 5, 5, 1, seed=1, ene=ene)
+    c = g.synth_complex(1200, 24, 5, 5, 1, seed=2, ene=ene)
+    d = a.desc
+    assert (d.prt_npoint, d.lig_natom, d.n_lig, d.n_prt, d.n_tmp) == (1200, 24, 5, 5, 1)
+    assert d.kde_npoint == 120 * 24 and d.mcs_nrow == 64
+    A, B, Cc = a.arrays(), b.arrays(), c.arrays()
+    for k in ("prt_x", "lig_x", "kde_x", "mcs_x", "prt_t", "lig_t"):
+        assert np.array_equal(A[k], B[k]), k
+    assert not np.array_equal(A["prt_x"], Cc["prt_x"])
+    # protein attributes follow the loader's typing rules
+    assert abs(float(A["prt_ele"].sum())) < 1e-3
+    cdc = (A["prt_c"] == 2) | ((A["prt_c"] == 0) & np.isin(A["prt_t"], [0]) & False)
+    assert np.all(A["prt_cdc"][A["prt_c"] == 2] == 1)
+    assert np.all(np.diff(A["prt_c"]) >= 0), "points sorted by class"
+    assert np.all(np.diff(A["lig_t"]) >= 0), "atoms sorted by type"
+    assert set(np.unique(A["lig_t"])) <= {2, 3, 4, 11, 13, 14, 16, 17, 18}
+    # ligand conformers are centred on their own centroid
+    assert np.abs(A["lig_x"].mean(1)).max() < 1e-4
+    # every MCS row matches at least one atom
+    assert np.all((A["mcs_x"] != np.float32(abi.GD_MCS_INVALID)).sum(1) >= 1)
+    assert cdc is not None
+
+
+def test_protein_typing_rules(golden_job):
+    """Effective points of 1a07C-1.pdb: 104 residues -> 344 points by the
+    loader's expansion (CA, PP from the third residue on, side chains)."""
+    A = golden_job.arrays()
+    t, c = A["prt_t"], A["prt_c"]
+    assert (c == 0).sum() == 104            # one CA per residue
+    assert (c == 1).sum() == 102            # PP for residues 3..104
+    assert np.all(t[c == 0] == 0) and np.all(t[c == 1] == 1)
+    assert np.all((t[c >= 2] >= 2) & (t[c >= 2] <= 29))
+    assert np.all(A["prt_cdc"][c == 2] == 1) and np.all(A["prt_cdc"][c == 1] == 0)
+    assert np.all(np.diff(c) >= 0)

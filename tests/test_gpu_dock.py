@@ -1,0 +1,253 @@
+"""GPU Monte Carlo path against the CPU oracle: initial pass, trajectories,
+acceptance statistics, temperature exchange, records, batching (persistent
+work queue + graph-captured segments) and the serving loop.
+
+Trajectory parity: both sides draw the same Philox numbers, so the accept
+decisions agree step for step until a rounding-level energy difference lands
+on the Metropolis threshold; after that the chains decorrelate and only
+their statistics (acceptance ratio, best-energy distribution) are compared.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import geauxdock_amd as g
+from geauxdock_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_job(ene):
+    # C1-like shape, reduced protein: 5 ligand x 5 protein conformers x 2 temperatures
+    return g.synth_complex(600, 24, 5, 5, 2, floor_temp=0.3, ceiling_temp=0.6, seed=1, ene=ene)
+
+
+def test_initial_pass_matches_oracle(golden_job, oracle):
+    dev = g.DeviceJob(golden_job)
+    mc = g.make_mc(0, move_mode="reference")
+    out, _ = dev.dock(mc)
+    o_out, _, _ = oracle.dock(golden_job, mc, threads=4)
+    # every replica starts at its conformer's native pose: forced accept of the initial energy
+    assert np.all(out.is_accept == 1)
+    err = np.abs(out.energy - o_out.energy).max(0)
+    assert err.max() < 5e-5, err
+    # replica 0 reproduces the published line
+    golden = [0.6434, -0.0367, -0.2079, -0.1845, 0.8521, -0.8880, 0.0523, 0.1744, -1.0, 0.7735]
+    assert " ".join("%.4f" % x for x in out.energy[0]) == " ".join("%.4f" % x for x in golden)
+
+
+@pytest.mark.parametrize("mode", ["reference", "gaussian"])
+def test_short_trajectory_matches_oracle(small_job, oracle, mode):
+    dev = g.DeviceJob(small_job)
+    mc = g.make_mc(30, move_mode=mode, seed=7)
+    out, _ = dev.dock(mc)
+    o_out, _, _ = oracle.dock(small_job, mc, threads=8)
+    same = (out.n_accept == o_out.n_accept) & (np.abs(out.movematrix - o_out.movematrix).max(1) < 1e-3)
+    assert same.mean() >= 0.9, same.mean()
+    # where the chains agree the accepted energies agree to FP32 rounding
+    e = np.abs(out.energy[same] - o_out.energy[same]).max()
+    assert e < 1e-3, e
+
+
+def test_acceptance_statistics_match_oracle(ene, oracle):
+    job = g.synth_complex(400, 16, 4, 4, 4, floor_temp=0.05, ceiling_temp=0.5, seed=3, ene=ene)
+    dev = g.DeviceJob(job)
+    mc = g.make_mc(400, move_mode="gaussian", seed=11)
+    out, _ = dev.dock(mc)
+    o_out, _, _ = oracle.dock(job, mc, threads=8)
+    n = 400 * job.n_rep
+    pa, pb = out.n_accept.sum() / n, o_out.n_accept.sum() / n
+    se = math.sqrt(pa * (1 - pa) / n + pb * (1 - pb) / n) * 4  # chains are correlated: 4x margin
+    assert abs(pa - pb) < 4 * se + 1e-3, (pa, pb)
+    from scipy import stats
+    ks = stats.ks_2samp(out.best_energy, o_out.best_energy)
+    assert ks.pvalue > 1e-3, ks
+
+
+def test_exchange(ene, oracle):
+    job = g.synth_complex(400, 16, 3, 2, 6, floor_temp=0.004, ceiling_temp=0.04, seed=5, ene=ene)
+    dev = g.DeviceJob(job)
+    mc = g.make_mc(200, steps_per_exchange=10, move_mode="gaussian", seed=13)
+    out, _ = dev.dock(mc)
+    nl, np_, nt = 3, 2, 6
+    # each (ligand conformer, protein conformer) ladder holds every temperature once
+    for p in range(np_):
+        for l in range(nl):
+            reps = [nt * nl * p + nl * t + l for t in range(nt)]
+            assert sorted(out.idx_tmp[reps]) == list(range(nt))
+    assert out.n_exchange.sum() > 0
+    assert out.n_exchange.sum() % 2 == 0
+    o_out, _, _ = oracle.dock(job, mc, threads=8)
+    # swap counts agree statistically with the oracle's chain
+    a, b = out.n_exchange.sum(), o_out.n_exchange.sum()
+    assert abs(a - b) <= 6 * math.sqrt(max(a, b)) + 4, (a, b)
+
+
+def test_records(small_job):
+    dev = g.DeviceJob(small_job)
+    mc = g.make_mc(60, steps_per_dump=20, move_mode="reference", seed=3, record=True)
+    out, _ = dev.dock(mc)
+    rec, cnt = dev.records(20)
+    # the last dump covers steps 40..59; a record is the state before a step
+    # whose predecessor was accepted
+    assert np.all(cnt <= 20) and cnt.sum() > 0
+    for r in range(small_job.n_rep):
+        steps = rec[r, :cnt[r], 0]
+        assert np.all(np.diff(steps) > 0) and np.all((steps >= 40) & (steps < 60))
+
+
+def test_dock_is_deterministic(small_job):
+    dev = g.DeviceJob(small_job)
+    mc = g.make_mc(20, move_mode="gaussian", seed=21)
+    a, _ = dev.dock(mc)
+    b, _ = dev.dock(mc)
+    assert np.array_equal(a.energy, b.energy) and np.array_equal(a.movematrix, b.movematrix)
+
+
+def _batch(jobs, mcs, stats=False):
+    import ctypes as C
+    outs = [g.host.ReplicaOut(j.n_rep) for j in jobs]
+    arr = (C.c_void_p * len(jobs))(*[j.h.value for j in jobs])
+    mca = (abi.gd_mc_param * len(jobs))(*mcs)
+    oa = (abi.gd_replica_out * len(jobs))(*[o.c for o in outs])
+    st = abi.gd_stats()
+    g.check(g.lib().gd_batch_dock(arr, len(jobs), mca, oa, C.byref(st) if stats else None),
+            "gd_batch_dock")
+    return outs
+
+
+def test_batch_matches_plain_bitwise(ene):
+    """A job computes the same bits alone (gd_dock) and inside a batch
+    (persistent work-queue pair kernel, graph-replayed segments)."""
+    hjobs = [g.synth_complex(P, L, 2, 3, 2, seed=s, ene=ene)
+             for P, L, s in [(500, 12, 1), (900, 30, 2), (1600, 20, 3)]]
+    mcs = [g.make_mc(120, steps_per_exchange=20, move_mode="gaussian", seed=5, job_id=k)
+           for k in range(3)]
+    plain = []
+    for h, mc in zip(hjobs, mcs):
+        d = g.DeviceJob(h)
+        d.set_tiling(8, d.tiling()[1])
+        plain.append(d.dock(mc, stats=False)[0])
+    devs = [g.DeviceJob(h) for h in hjobs]
+    for d in devs:
+        d.set_tiling(8, d.tiling()[1])
+    for rep in range(3):  # several graph replays, eager work in between
+        outs = _batch(devs, mcs)
+        for a, b in zip(plain, outs):
+            assert np.array_equal(a.energy, b.energy)
+            assert np.array_equal(a.movematrix, b.movematrix)
+            assert np.array_equal(a.idx_tmp, b.idx_tmp)
+        devs[0].eval(np.zeros((1, 2), np.int32), np.zeros((1, 6), np.float32))
+
+
+def test_batch_timed_path_matches_graph_path(ene):
+    hjobs = [g.synth_complex(700, 16, 2, 2, 1, seed=s, ene=ene) for s in (4, 5)]
+    mcs = [g.make_mc(100, move_mode="gaussian", seed=9, job_id=k) for k in range(2)]
+    devs = [g.DeviceJob(h) for h in hjobs]
+    a = _batch(devs, mcs, stats=False)
+    b = _batch(devs, mcs, stats=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x.energy, y.energy)
+
+
+def test_server_alone_vs_mix(ene):
+    """Continuous batching: a request admitted while others run returns the
+    same result as when it runs alone."""
+    import ctypes as C
+    target = g.synth_complex(800, 20, 3, 2, 2, seed=31, ene=ene)
+    others = [g.synth_complex(P, L, 2, 2, 2, seed=s, ene=ene)
+              for P, L, s in [(1500, 40, 32), (600, 12, 33), (1200, 60, 34)]]
+    mc_t = g.make_mc(150, steps_per_exchange=25, move_mode="gaussian", seed=77, job_id=1000)
+
+    def serve(jobs_and_mc, sweep):
+        cfg = abi.gd_server_config(hbm_budget=64 << 30, max_active=8, sweep_steps=sweep)
+        s = C.c_void_p()
+        g.check(g.lib().gd_server_create(C.byref(cfg), C.byref(s)), "gd_server_create")
+        outs = {}
+        try:
+            pending = list(jobs_and_mc)
+            done = 0
+            while done < len(jobs_and_mc):
+                if pending:  # one arrival per sweep: later jobs join a running batch
+                    h, mc, key = pending.pop(0)
+                    o = g.host.ReplicaOut(h.n_rep)
+                    t = C.c_int64()
+                    g.check(g.lib().gd_server_submit(s, C.byref(h.desc), C.byref(mc), C.byref(o.c),
+                                                     C.byref(t)), "submit")
+                    outs[key] = o
+                na, nd = C.c_int(), C.c_int()
+                g.check(g.lib().gd_server_step(s, C.byref(na), C.byref(nd)), "step")
+                res = (abi.gd_job_result * 16)()
+                n = C.c_int()
+                g.lib().gd_server_results(s, res, 16, C.byref(n))
+                for i in range(n.value):
+                    assert res[i].status == 0
+                    assert res[i].t_done >= res[i].t_admit >= res[i].t_submit
+                done += n.value
+        finally:
+            g.lib().gd_server_destroy(s)
+        return outs
+
+    alone = serve([(target, mc_t, "t")], sweep=40)["t"]
+    mix = [(others[0], g.make_mc(300, move_mode="gaussian", seed=1, job_id=1), "a"),
+           (others[1], g.make_mc(90, move_mode="gaussian", seed=2, job_id=2), "b"),
+           (target, mc_t, "t"),
+           (others[2], g.make_mc(200, steps_per_exchange=10, move_mode="gaussian", seed=3, job_id=3), "c")]
+    mixed = serve(mix, sweep=17)["t"]
+    assert np.array_equal(alone.energy, mixed.energy)
+    assert np.array_equal(alone.movematrix, mixed.movematrix)
+    assert np.array_equal(alone.idx_tmp, mixed.idx_tmp)
+
+
+def test_server_admission_budget(ene):
+    """A request larger than the whole budget is refused; requests that fit
+    only one at a time run one after another."""
+    import ctypes as C
+    h = g.synth_complex(800, 20, 2, 2, 2, seed=41, ene=ene)
+    foot = g.lib().gd_job_footprint(C.byref(h.desc))
+    assert foot > 0
+    cfg = abi.gd_server_config(hbm_budget=foot + foot // 2, max_active=8, sweep_steps=10)
+    s = C.c_void_p()
+    g.check(g.lib().gd_server_create(C.byref(cfg), C.byref(s)), "create")
+    try:
+        big = g.synth_complex(4000, 40, 8, 4, 4, seed=42, ene=ene)
+        t = C.c_int64()
+        mc = g.make_mc(20, seed=1)
+        assert g.lib().gd_server_submit(s, C.byref(big.desc), C.byref(mc), None, C.byref(t)) == abi.GD_ERR_BUSY
+        outs = [g.host.ReplicaOut(h.n_rep) for _ in range(3)]
+        for k in range(3):
+            g.check(g.lib().gd_server_submit(s, C.byref(h.desc), C.byref(mc), C.byref(outs[k].c),
+                                             C.byref(t)), "submit")
+        na, nd = C.c_int(), C.c_int()
+        g.check(g.lib().gd_server_step(s, C.byref(na), C.byref(nd)), "step")
+        used, pend = C.c_size_t(), C.c_int()
+        g.lib().gd_server_usage(s, C.byref(used), C.byref(pend))
+        assert na.value <= 1 and used.value <= cfg.hbm_budget
+        total = nd.value
+        for _ in range(20):
+            g.check(g.lib().gd_server_step(s, C.byref(na), C.byref(nd)), "step")
+            total = nd.value
+            if total == 3:
+                break
+        assert total == 3
+        # identical requests, identical results
+        assert np.array_equal(outs[0].energy, outs[1].energy)
+    finally:
+        g.lib().gd_server_destroy(s)
+
+
+def test_autotune_keeps_parity(ene, oracle):
+    job = g.synth_complex(1200, 24, 2, 2, 2, seed=51, ene=ene)
+    dev = g.DeviceJob(job)
+    import ctypes as C
+    t = abi.gd_tiling()
+    ms = C.c_double()
+    g.check(g.lib().gd_autotune(dev.h, 3, C.byref(t), C.byref(ms)), "gd_autotune")
+    assert t.atoms_per_lane in (4, 8, 16) and t.tile_points > 0 and ms.value > 0
+    from tests.oracle_ffi import random_poses, rel_err
+    conf, pose = random_poses(job, 32, seed=2)
+    _, raw, coords = dev.eval(conf, pose, want_coords=True)
+    _, r_raw, r_scale, _ = oracle.energy(job, conf, pose, mode=2, coords=coords)
+    assert rel_err(raw, r_raw, r_scale).max() <= 1e-5
