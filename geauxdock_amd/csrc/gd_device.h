@@ -51,7 +51,7 @@ struct GdJob {
   const GdAtom *atoms;    // [L]
   const float4 *pcoord;   // [n_prt][P]
   const float4 *pattr;    // [P] (ele, hpp, flags-as-int, 0)
-  const float *ptab;      // [n_types][GD_PTAB_PT][GD_PTAB_W]
+  const float *ptab;      // [n_types][GD_PTAB_PT][GD_PTAB_W]: p1a p2a pmf0 pmf1 hdb1c hdb0c hdb1 psp-threshold
   const float *pspv;      // [n_types][P] psp value of class-2 points
   const float4 *kde;      // [K] sorted by type
   const float4 *mcs;      // [M][L] (x, y, z, valid)

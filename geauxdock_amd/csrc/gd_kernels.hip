@@ -102,7 +102,10 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
     hpc[j] = 0.0f;
   }
   const GD_AS_GLOBAL float4 *__restrict__ pc = gd_g(J.pcoord) + (size_t)gd_g(J.prti)[r] * P;
-  const float lj0 = J.lj0, lj1 = J.lj1, el0 = J.el0, el1 = J.el1, a1 = J.a1, b1 = J.b1;
+  // LJ denominator p1*lj0*(1 + lj1 r^2) + 1 = fma(p1, lj0 + lj0*lj1*r^2, 1);
+  // electrostatic polynomial el0 + a1 s^2 + b1 s^3 with s = el1 r, written in r
+  const float lj0 = J.lj0, lj01 = J.lj0 * J.lj1;
+  const float el0 = J.el0, ea = J.a1 * J.el1 * J.el1, eb = J.b1 * J.el1 * J.el1 * J.el1;
   const float inv_el1 = J.inv_el1;
   float evdw = 0.0f, epmf = 0.0f, epsp = 0.0f, ehdb = 0.0f;
 
@@ -119,7 +122,7 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
       if (j < nl) {
         const GD_AS_CONST float *__restrict__ prm = ptab + (size_t)(tid[j] + pt) * GD_PTAB_W;
         const float p1a = prm[0], p2a = prm[1], pmf0 = prm[2], pmf1 = prm[3];
-        const float hdb0 = prm[4], hdb1c = prm[5], hdb1 = prm[6], pthr = prm[7];
+        const float hdb1c = prm[4], hdb0c = prm[5], hdb1 = prm[6], pthr = prm[7];
         const float dx = lx[j] - c.x, dy = ly[j] - c.y, dz = lz[j] - c.z;
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         const float ri = gd_rsq(r2);
@@ -141,14 +144,13 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
           const float ri9 = ri6 * ri3;
           const float q1 = p1a * ri9;
           const float q2 = p2a * ri6;
-          const float den = __builtin_fmaf(q1 * lj0, __builtin_fmaf(lj1, r2, 1.0f), 1.0f);
+          const float den = __builtin_fmaf(q1, __builtin_fmaf(lj01, r2, lj0), 1.0f);
           evdw = __builtin_fmaf(q1 - q2, gd_rcp(den), evdw);
         }
         // distance-dependent electrostatics (:72-78), charge applied per atom
         {
-          const float s = el1 * rr;
-          const float g = (s < 1.0f) ? __builtin_fmaf(__builtin_fmaf(b1, s, a1), s * s, el0)
-                                     : ri * inv_el1;
+          const float poly = __builtin_fmaf(__builtin_fmaf(eb, rr, ea), r2, el0);
+          const float g = (rr < inv_el1) ? poly : ri * inv_el1;
           ele[j] = __builtin_fmaf(pele, g, ele[j]);
         }
         // contact PMF (:88-91)
@@ -163,9 +165,12 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
           const float v = pspv[(size_t)(tid[j] / GD_PTAB_PT) * P + p];
           epsp += (r2 <= pthr) ? v : 0.0f;
         }
-        // hydrogen bond (:116-121); rows with hdb0 <= 0.1 carry hdb1 == 0
-        if (__float_as_int(hdb1) != 0) {
-          const float h = (rr - hdb0) * hdb1c;
+        // hydrogen bond (:116-121); rows with hdb0 <= 0.1 carry hdb1 == 0 and
+        // skip the term on a wave-uniform branch (kept a branch: the empty
+        // volatile asm stops the compiler from evaluating it for every pair)
+        if (((const GD_AS_CONST int *)prm)[6] != 0) {
+          asm volatile("");
+          const float h = __builtin_fmaf(rr, hdb1c, -hdb0c);
           ehdb = __builtin_fmaf(hdb1, gd_exp2(-(h * h)), ehdb);
         }
       }

@@ -232,9 +232,12 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       o[1] = e->p2a[lt][pt];
       o[2] = e->pmf0[lt][pt];
       o[3] = e->pmf1[lt][pt];
+      // H-bond: exp(-(r - hdb0)^2 hdb1^2 / 2) = exp2(-(r c - hdb0 c)^2),
+      // c = hdb1 sqrt(log2(e)/2); rows with hdb0 <= 0.1 are off (hdb1 = 0)
       const bool hb = e->hdb0[lt][pt] > 0.1f;
-      o[4] = hb ? e->hdb0[lt][pt] : 0.0f;
-      o[5] = hb ? e->hdb1[lt][pt] * hdb_c : 0.0f;
+      const float c = e->hdb1[lt][pt] * hdb_c;
+      o[4] = hb ? c : 0.0f;
+      o[5] = hb ? e->hdb0[lt][pt] * c : 0.0f;
       o[6] = hb ? e->hdb1[lt][pt] : 0.0f;
       o[7] = sqrt_threshold(e->pmf0[lt][pt]);
     }

@@ -50,6 +50,10 @@
 #define GD_REF_MAX_STR_LENG 256
 #define GD_REF_ALIGN __attribute__((aligned(64)))
 
+/* A translation unit that already included the reference's geauxdock.h
+ * (guard DOCK_H) uses the reference's own definitions: same bytes. */
+#ifndef DOCK_H
+
 struct GD_REF_ALIGN Ligand {
   float x1[GD_REF_MAXLIG], y1[GD_REF_MAXLIG], z1[GD_REF_MAXLIG];
   int t1[GD_REF_MAXLIG];
@@ -198,6 +202,8 @@ struct ParaT {
   int n_rep;
   int record_sz;
 };
+
+#endif /* DOCK_H */
 
 /* Counter-based RNG stream: Philox4x32-10 key and the job's stream id. */
 struct gd_rng_state {

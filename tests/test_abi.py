@@ -112,3 +112,21 @@ def test_protein_typing_rules(golden_job):
     assert np.all((t[c >= 2] >= 2) & (t[c >= 2] <= 29))
     assert np.all(A["prt_cdc"][c == 2] == 1) and np.all(A["prt_cdc"][c == 1] == 0)
     assert np.all(np.diff(c) >= 0)
+
+
+def test_dropin_layout_matches_reference():
+    """include/geauxdock_dropin.h reproduces sizeof/offsetof of the reference
+    data model (tests/golden/ref_layout.json, written by make_ref_layout.py
+    from the reference headers)."""
+    import json
+    import shutil
+    import sys
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_ref_layout as m
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_layout.json")))
+    mine = m.measure('#include "%s"' % os.path.join(ROOT, "include", "geauxdock_dropin.h"), [])
+    bad = {k: (mine.get(k), v) for k, v in ref.items() if mine.get(k) != v}
+    assert not bad, bad
+    assert len(ref) > 100
