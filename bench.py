@@ -67,21 +67,157 @@ def cpu_baseline(job, mc_template, budget_s=15.0):
                       "OpenMP over replicas" % (job.n_rep, steps)}
 
 
+def _sizes(rng, n, kind):
+    """Per-job sizes of the batched / served workloads (seeded)."""
+    out = []
+    for i in range(n):
+        L = int(rng.integers(12, 81))
+        if kind == "c3":
+            out.append(dict(P=4000, L=L, nlig=10, nprt=11, ntmp=4))
+        else:
+            P = int(round(np.exp(rng.uniform(np.log(1500), np.log(9000)))))
+            nlig = 5 if kind == "c4" else int(rng.integers(5, 21))
+            out.append(dict(P=P, L=L, nlig=nlig, nprt=11, ntmp=4))
+    return out
+
+
+def batched(kind, n_jobs, mc_steps, plain_jobs):
+    """Batched form (one launch sequence, persistent work-queue pair kernel,
+    graph-captured segments) vs the plain one-job-per-launch path on the
+    same complexes; jobs/s of each."""
+    import geauxdock_amd as g
+    from geauxdock_amd import abi
+    rng = np.random.default_rng(3 if kind == "c3" else 4)
+    sizes = _sizes(rng, n_jobs, kind)
+    hosts = [g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.004,
+                             ceiling_temp=0.04, seed=100 + i) for i, z in enumerate(sizes)]
+    devs = [g.DeviceJob(h) for h in hosts]
+    for d in devs:                       # coarse batch tiling: 8 atoms x <= 1024 points
+        d.set_tiling(8, min(1024, d.host.desc.prt_npoint))
+    mcs = [g.make_mc(mc_steps, steps_per_exchange=100, move_mode="gaussian", seed=5, job_id=i)
+           for i in range(n_jobs)]
+    arr = (C.c_void_p * n_jobs)(*[d.h.value for d in devs])
+    mca = (abi.gd_mc_param * n_jobs)(*mcs)
+    t0 = time.perf_counter()
+    g.check(g.lib().gd_batch_dock(arr, n_jobs, mca, None, None), "gd_batch_dock")
+    tb = time.perf_counter() - t0
+    rep_steps = sum(h.n_rep for h in hosts) * mc_steps
+    # plain path: the first plain_jobs complexes one at a time, default tiling
+    t0 = time.perf_counter()
+    for i in range(plain_jobs):
+        dp = g.DeviceJob(hosts[i])
+        dp.dock(mcs[i], stats=False)
+        dp.close()
+    tp = time.perf_counter() - t0
+    return {"workload": kind, "jobs": n_jobs, "mc_steps": mc_steps,
+            "batched_jobs_per_s": n_jobs / tb, "batched_replica_steps_per_s": rep_steps / tb,
+            "plain_jobs_per_s": plain_jobs / tp,
+            "speedup_jobs_per_s": (n_jobs / tb) / (plain_jobs / tp), "plain_sample_jobs": plain_jobs}
+
+
+def serve(rates, n_jobs, step_scale):
+    """Serving: Poisson arrivals, admission by device footprint, continuous
+    batching; p50/p99 latency (arrival -> result) and jobs/s per rate, and the
+    plain path's jobs/s on the same requests."""
+    import geauxdock_amd as g
+    from geauxdock_amd import abi
+    res = []
+    rng = np.random.default_rng(5)
+    sizes = _sizes(rng, n_jobs, "c5")
+    steps = [int(rng.integers(2000, 10001) * step_scale) for _ in range(n_jobs)]
+    hosts = [g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.004,
+                             ceiling_temp=0.04, seed=500 + i) for i, z in enumerate(sizes)]
+    mcs = [g.make_mc(steps[i], steps_per_exchange=100, move_mode="gaussian", seed=7, job_id=i)
+           for i in range(n_jobs)]
+    for lam in rates:
+        arr_t = np.cumsum(np.random.default_rng(int(lam)).exponential(1.0 / lam, n_jobs))
+        cfg = abi.gd_server_config(hbm_budget=200 << 30, max_active=256, sweep_steps=50)
+        s = C.c_void_p()
+        g.check(g.lib().gd_server_create(C.byref(cfg), C.byref(s)), "gd_server_create")
+        t0 = time.perf_counter()
+        sub, done, lat = 0, 0, []
+        submit_t = {}
+        while done < n_jobs:
+            now = time.perf_counter() - t0
+            while sub < n_jobs and arr_t[sub] <= now:
+                t = C.c_int64()
+                g.check(g.lib().gd_server_submit(s, C.byref(hosts[sub].desc), C.byref(mcs[sub]), None,
+                                                 C.byref(t)), "submit")
+                submit_t[t.value] = arr_t[sub]
+                sub += 1
+            na, nd = C.c_int(), C.c_int()
+            g.check(g.lib().gd_server_step(s, C.byref(na), C.byref(nd)), "gd_server_step")
+            rr = (abi.gd_job_result * 64)()
+            n = C.c_int()
+            g.lib().gd_server_results(s, rr, 64, C.byref(n))
+            end = time.perf_counter() - t0
+            for k in range(n.value):
+                lat.append(end - submit_t[rr[k].ticket])
+            done += n.value
+            if na.value == 0 and sub < n_jobs:
+                time.sleep(max(0.0, arr_t[sub] - (time.perf_counter() - t0)))
+        wall = time.perf_counter() - t0
+        g.lib().gd_server_destroy(s)
+        lat = np.array(lat)
+        res.append({"rate_per_s": lam, "jobs": n_jobs, "jobs_per_s": n_jobs / wall,
+                    "p50_latency_s": float(np.percentile(lat, 50)),
+                    "p99_latency_s": float(np.percentile(lat, 99))})
+    t0 = time.perf_counter()
+    for i in range(n_jobs):
+        d = g.DeviceJob(hosts[i])
+        d.dock(mcs[i], stats=False)
+        d.close()
+    tp = time.perf_counter() - t0
+    return {"workload": "c5-serving", "step_scale": step_scale, "per_rate": res,
+            "plain_jobs_per_s": n_jobs / tp,
+            "plain_note": "same requests, one job per launch sequence, back to back"}
+
+
+def copy_calibration(n_bytes=256 << 20, reps=3):
+    """A plain device-to-device copy of known size: the HBM counters
+    (FETCH_SIZE, WRITE_SIZE) of its dispatches against n_bytes give the scale
+    of those counters on this device."""
+    import torch
+    x = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    return {"workload": "copy-calibration", "bytes_read_per_copy": n_bytes,
+            "bytes_written_per_copy": n_bytes, "copies": reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100, help="exchange intervals (100 MC steps each)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "copy"], default="c2",
+                    help="c2 = the headline line; c3/c4 batched vs plain; c5 serving (extra lines); "
+                         "copy = a plain 256 MiB device copy, the calibration for HBM counters")
+    ap.add_argument("--jobs", type=int, default=32)
+    ap.add_argument("--mc-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(jobs shard across GPUs without communication, see DESIGN.md)" % args.gpus,
+              "(jobs and (ligand, protein) replica groups shard across GPUs without communication)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        print(json.dumps(copy_calibration()), flush=True)
+        return
     import geauxdock_amd as g
     from geauxdock_amd import abi
     g.check(g.lib().gd_init(0), "gd_init")
+    if args.workload in ("c3", "c4"):
+        print(json.dumps(batched(args.workload, args.jobs, args.mc_steps, max(1, args.jobs // 8))),
+              flush=True)
+        return
+    if args.workload == "c5":
+        print(json.dumps(serve([50, 200, 800], args.jobs, args.mc_steps / 10000.0)), flush=True)
+        return
     name = C.create_string_buffer(128)
     ncu, clk, hbm = C.c_int(), C.c_int(), C.c_size_t()
     g.lib().gd_device_info(name, 128, C.byref(ncu), C.byref(clk), C.byref(hbm))

@@ -108,6 +108,7 @@ PROTOTYPES = {
     "gd_dock": (C.c_int, [_vp, C.POINTER(gd_mc_param), C.POINTER(gd_replica_out),
                           C.POINTER(gd_stats)]),
     "gd_dock_records": (C.c_int, [_vp, C.c_int, _fp, _ip]),
+    "gd_job_replicas": (C.c_int, [_vp, C.POINTER(gd_replica_out)]),
     "gd_batch_dock": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(gd_mc_param),
                                 C.POINTER(gd_replica_out), C.POINTER(gd_stats)]),
     "gd_tiling_get": (C.c_int, [_vp, C.POINTER(gd_tiling)]),

@@ -42,6 +42,8 @@ __device__ __forceinline__ float gd_exp2(float x) { return __builtin_amdgcn_exp2
 #define GD_HPB (4.5f / 81.0f / 81.0f)
 #define GD_HPC (2.5f / 81.0f / 81.0f / 81.0f)
 #define GD_HPD (0.5f / 81.0f / 81.0f / 81.0f / 81.0f)
+// 1 / sqrt(log2(e) / 2) = sqrt(2 ln 2): H-bond sums carry the factor sqrt(log2(e) / 2)
+#define GD_HDB_RC 1.17741002251547469f
 
 // ---------------------------------------------------------------------------
 // Address-space helpers: tables shared by all lanes are read through the
@@ -72,7 +74,7 @@ __device__ __forceinline__ const GD_AS_GLOBAL T *gd_g(const T *p) {
 
 // ---------------------------------------------------------------------------
 // Pair kernel.  LC ligand atoms per wave, all LC coordinates in registers.
-template <int LC>
+template <int LC, int EMIN>
 __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, const GdItem it,
                                              const int lane) {
   const GD_AS_CONST GdJob &J = gd_c(jobs)[it.job];
@@ -91,6 +93,7 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   // ligand chunk: coordinates of this lane's replica, per-atom constants (uniform)
   float lx[LC], ly[LC], lz[LC], ele[LC], hpc[LC];
   int tid[LC];
+  unsigned hmask[LC];
 #pragma unroll
   for (int j = 0; j < LC; ++j) {
     const int l = l0 + min(j, nl - 1);
@@ -98,15 +101,20 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
     ly[j] = coords[(size_t)(3 * l + 1) * Rp + r];
     lz[j] = coords[(size_t)(3 * l + 2) * Rp + r];
     tid[j] = atoms[l].tidx * GD_PTAB_PT;
+    hmask[j] = atoms[l].hmask;
     ele[j] = 0.0f;
     hpc[j] = 0.0f;
   }
   const GD_AS_GLOBAL float4 *__restrict__ pc = gd_g(J.pcoord) + (size_t)gd_g(J.prti)[r] * P;
-  // LJ denominator p1*lj0*(1 + lj1 r^2) + 1 = fma(p1, lj0 + lj0*lj1*r^2, 1);
-  // electrostatic polynomial el0 + a1 s^2 + b1 s^3 with s = el1 r, written in r
   const float lj0 = J.lj0, lj01 = J.lj0 * J.lj1;
-  const float el0 = J.el0, ea = J.a1 * J.el1 * J.el1, eb = J.b1 * J.el1 * J.el1 * J.el1;
+  // electrostatics carried as el1 * g: el1 (el0 + a1 s^2 + b1 s^3) with s = el1 r
+  // is a polynomial in r, and el1 / s = 1 / r; the sum is scaled by 1/el1 once
+  const float el1 = J.el1;
+  const float e0 = J.el0 * el1, e2 = J.a1 * el1 * el1 * el1, e3 = J.b1 * el1 * el1 * el1 * el1;
   const float inv_el1 = J.inv_el1;
+  // -log2(e)/2 held in a VGPR: a VALU op reads at most one SGPR or literal
+  float pmf_a;
+  asm("v_mov_b32 %0, 0xbf38aa3b" : "=v"(pmf_a));
   float evdw = 0.0f, epmf = 0.0f, epsp = 0.0f, ehdb = 0.0f;
 
   float4 cn = pc[p0 < p1 ? p0 : 0];
@@ -121,42 +129,47 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
     for (int j = 0; j < LC; ++j) {
       if (j < nl) {
         const GD_AS_CONST float *__restrict__ prm = ptab + (size_t)(tid[j] + pt) * GD_PTAB_W;
-        const float p1a = prm[0], p2a = prm[1], pmf0 = prm[2], pmf1 = prm[3];
-        const float hdb1c = prm[4], hdb0c = prm[5], hdb1 = prm[6], pthr = prm[7];
+        const float p1a = prm[0], np2a = prm[1], pB = prm[2], pC = prm[3];
+        const float pmf1 = prm[4], pthr = prm[5], hdb1c = prm[6], nhdb0c = prm[7];
         const float dx = lx[j] - c.x, dy = ly[j] - c.y, dz = lz[j] - c.z;
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         const float ri = gd_rsq(r2);
         const float rr = r2 * ri;
-        // hydrophobic contact (energy_prt_v1.C:44-50)
+        // hydrophobic contact (energy_prt_v1.C:44-50):
+        // hpp (1 - r^2 (a - b r^2 + c r^4 - d r^6)) for r^2 <= 81
+        // (the empty volatile asm keeps this a wave-uniform branch; without it
+        // the compiler evaluates the term for every point and selects)
         if (fl & GD_PF_CDC) {
-          float t = __builtin_fmaf(-GD_HPD, r2, GD_HPC);
-          t = __builtin_fmaf(t, r2, -GD_HPB);
-          t = __builtin_fmaf(t, r2, GD_HPA);
-          t *= r2;
-          const float h = __builtin_fmaf(-phpp, t, phpp);
-          hpc[j] += (r2 <= 81.0f) ? h : 0.0f;
+          asm volatile("");
+          float w = __builtin_fmaf(-GD_HPD, r2, GD_HPC);
+          w = __builtin_fmaf(w, r2, -GD_HPB);
+          w = __builtin_fmaf(w, r2, GD_HPA);
+          const float v = __builtin_fmaf(-r2, w, 1.0f);
+          hpc[j] = __builtin_fmaf(phpp, (r2 <= 81.0f) ? v : 0.0f, hpc[j]);
         }
-        // soft Lennard-Jones (:61-64) with 1/r^6, 1/r^9 from one rsq
+        // soft Lennard-Jones (:61-64): (p1 - p2) / (p1 lj0 (1 + lj1 r^2) + 1) with
+        // p1 = p1a / r^9, p2 = p2a / r^6, multiplied through by r^9:
+        // (p1a - p2a r^3) / (p1a lj0 (1 + lj1 r^2) + r^9)
         {
-          const float ri2 = ri * ri;
-          const float ri3 = ri2 * ri;
-          const float ri6 = ri3 * ri3;
-          const float ri9 = ri6 * ri3;
-          const float q1 = p1a * ri9;
-          const float q2 = p2a * ri6;
-          const float den = __builtin_fmaf(q1, __builtin_fmaf(lj01, r2, lj0), 1.0f);
-          evdw = __builtin_fmaf(q1 - q2, gd_rcp(den), evdw);
+          const float r3 = r2 * rr;
+          const float r9 = (r3 * r3) * r3;
+          const float num = __builtin_fmaf(np2a, r3, p1a);
+          const float den = __builtin_fmaf(p1a, __builtin_fmaf(lj01, r2, lj0), r9);
+          evdw = __builtin_fmaf(num, gd_rcp(den), evdw);
         }
-        // distance-dependent electrostatics (:72-78), charge applied per atom
+        // distance-dependent electrostatics (:72-78), charge applied per atom.
+        // EMIN: the host checked that s·poly(s) - 1 is <= 0 below s = 1 and >= 0
+        // above it (true of the reference parameters, where it is (s-1)^3 (s+1)),
+        // so the branch on s < 1 is the smaller of the two forms
         {
-          const float poly = __builtin_fmaf(__builtin_fmaf(eb, rr, ea), r2, el0);
-          const float g = (rr < inv_el1) ? poly : ri * inv_el1;
+          const float poly = __builtin_fmaf(__builtin_fmaf(e3, rr, e2), r2, e0);
+          const float g = EMIN ? fminf(poly, ri) : ((rr < inv_el1) ? poly : ri);
           ele[j] = __builtin_fmaf(pele, g, ele[j]);
         }
-        // contact PMF (:88-91)
-        const float dmp = rr - pmf0;
+        // contact PMF (:88-91): exponent (6 - r/2)(r - pmf0) log2(e)
+        // = -log2(e)/2 r^2 + pB r + pC
         {
-          const float arg = __builtin_fmaf(-0.5f * GD_LOG2E, rr, 6.0f * GD_LOG2E) * dmp;
+          const float arg = __builtin_fmaf(pB, rr, __builtin_fmaf(pmf_a, r2, pC));
           epmf = __builtin_fmaf(pmf1, gd_rcp(1.0f + gd_exp2(arg)), epmf);
         }
         // pocket-specific potential (:100-107): r <= pmf0 decided on r^2
@@ -165,19 +178,20 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
           const float v = pspv[(size_t)(tid[j] / GD_PTAB_PT) * P + p];
           epsp += (r2 <= pthr) ? v : 0.0f;
         }
-        // hydrogen bond (:116-121); rows with hdb0 <= 0.1 carry hdb1 == 0 and
-        // skip the term on a wave-uniform branch (kept a branch: the empty
-        // volatile asm stops the compiler from evaluating it for every pair)
-        if (((const GD_AS_CONST int *)prm)[6] != 0) {
+        // hydrogen bond (:116-121) as c exp2(-(r c - hdb0 c)^2), c = hdb1 sqrt(log2(e)/2),
+        // summed without the 1/sqrt(log2(e)/2) factor; pairs with hdb0 <= 0.1 are
+        // off in the atom's point-type mask and skip the term on a wave-uniform branch
+        if ((hmask[j] >> pt) & 1u) {
           asm volatile("");
-          const float h = __builtin_fmaf(rr, hdb1c, -hdb0c);
-          ehdb = __builtin_fmaf(hdb1, gd_exp2(-(h * h)), ehdb);
+          const float h = __builtin_fmaf(rr, hdb1c, nhdb0c);
+          ehdb = __builtin_fmaf(hdb1c, gd_exp2(-(h * h)), ehdb);
         }
       }
     }
   }
+  ehdb *= GD_HDB_RC;
 
-  // KDE: this tile's slice of every chunk atom's anchor bin
+  // KDE: this tile's share of every chunk atom's anchor bin
   float ekde = 0.0f;
   const float kde2l = J.kde2l;
   const GD_AS_CONST float4 *__restrict__ kde = gd_c(J.kde);
@@ -185,11 +199,8 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   for (int j = 0; j < LC; ++j) {
     if (j < nl) {
       const GdAtom at = atoms[l0 + j];
-      const int nb = at.kde_e - at.kde_b;
-      const int k0 = at.kde_b + (int)(((long long)nb * it.b) / nP);
-      const int k1 = at.kde_b + (int)(((long long)nb * (it.b + 1)) / nP);
       float s = 0.0f;
-      for (int k = k0; k < k1; ++k) {
+      for (int k = at.kde_b + it.b; k < at.kde_e; k += nP) {  // tile b: bins b, b + nP, ...
         const float4 kp = kde[k];
         const float dx = lx[j] - kp.x, dy = ly[j] - kp.y, dz = lz[j] - kp.z;
         const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
@@ -199,14 +210,12 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
     }
   }
 
-  // MCS: this tile's slice of the rows, restricted to the chunk atoms
+  // MCS: this tile's share of the rows, restricted to the chunk atoms
   {
     const int M = J.M, nL = J.nL;
     const GD_AS_CONST float4 *__restrict__ mcs = gd_c(J.mcs);
     GD_AS_GLOBAL float *__restrict__ pm = gd_g(J.part_mcs);
-    const int m0 = (int)(((long long)M * it.b) / nP);
-    const int m1 = (int)(((long long)M * (it.b + 1)) / nP);
-    for (int m = m0; m < m1; ++m) {
+    for (int m = it.b; m < M; m += nP) {  // tile b: rows b, b + nP, ...
       float d = 0.0f;
 #pragma unroll
       for (int j = 0; j < LC; ++j) {
@@ -227,6 +236,7 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
 #pragma unroll
   for (int j = 0; j < LC; ++j)
     if (j < nl) eele = __builtin_fmaf(atoms[l0 + j].q, ele[j], eele);
+  eele *= inv_el1;
   const int nS = J.nL * nP;
   const int slot = it.a * nP + it.b;
   GD_AS_GLOBAL float *__restrict__ pl = gd_g(J.part_lin) + (size_t)slot * Rp + r;
@@ -244,13 +254,13 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
 }
 
 // One wave per work item (grid = items / 4 workgroups of 4 waves).
-template <int LC>
+template <int LC, int EMIN>
 __global__ __launch_bounds__(256) void gd_pair_kernel(const GdJob *__restrict__ jobs,
                                                        const GdItem *__restrict__ items,
                                                        int n_items) {
   const int iid = gd_uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (iid >= n_items) return;
-  gd_pair_item<LC>(jobs, gd_c(items)[iid], threadIdx.x & 63);
+  gd_pair_item<LC, EMIN>(jobs, gd_c(items)[iid], threadIdx.x & 63);
 }
 
 // Persistent form: a fixed grid sized to the machine; every wave pulls item
@@ -258,7 +268,7 @@ __global__ __launch_bounds__(256) void gd_pair_kernel(const GdJob *__restrict__ 
 // items of uneven cost (complexes of different sizes in one launch).  The
 // counter is zeroed by the kernel that runs before each launch (finish or
 // transform kernel); every wave exits once the counter passes n_items.
-template <int LC>
+template <int LC, int EMIN>
 __global__ __launch_bounds__(256) void gd_pair_kernel_wq(const GdJob *__restrict__ jobs,
                                                           const GdItem *__restrict__ items,
                                                           int n_items, int *__restrict__ wq) {
@@ -268,16 +278,19 @@ __global__ __launch_bounds__(256) void gd_pair_kernel_wq(const GdJob *__restrict
     if (lane == 0) iid = atomicAdd(wq, 1);
     iid = gd_uniform(__shfl(iid, 0));
     if (iid >= n_items) break;
-    gd_pair_item<LC>(jobs, gd_c(items)[iid], lane);
+    gd_pair_item<LC, EMIN>(jobs, gd_c(items)[iid], lane);
   }
 }
 
-template __global__ void gd_pair_kernel<4>(const GdJob *, const GdItem *, int);
-template __global__ void gd_pair_kernel<8>(const GdJob *, const GdItem *, int);
-template __global__ void gd_pair_kernel<16>(const GdJob *, const GdItem *, int);
-template __global__ void gd_pair_kernel_wq<4>(const GdJob *, const GdItem *, int, int *);
-template __global__ void gd_pair_kernel_wq<8>(const GdJob *, const GdItem *, int, int *);
-template __global__ void gd_pair_kernel_wq<16>(const GdJob *, const GdItem *, int, int *);
+#define GD_PAIR_INST(LC, EM)                                                              \
+  template __global__ void gd_pair_kernel<LC, EM>(const GdJob *, const GdItem *, int);    \
+  template __global__ void gd_pair_kernel_wq<LC, EM>(const GdJob *, const GdItem *, int, int *);
+GD_PAIR_INST(4, 0)
+GD_PAIR_INST(8, 0)
+GD_PAIR_INST(16, 0)
+GD_PAIR_INST(4, 1)
+GD_PAIR_INST(8, 1)
+GD_PAIR_INST(16, 1)
 
 // ---------------------------------------------------------------------------
 // Ligand transform of one replica's proposal (kernel_cpp_montecarlo_soa.C:
@@ -352,15 +365,38 @@ __device__ __forceinline__ void gd_move(const GdJob &J, int r, uint32_t step, fl
 }
 
 // ---------------------------------------------------------------------------
-// Finish kernel: 256 threads = 4 waves per 64-replica block.
-__global__ __launch_bounds__(256) void gd_finish_kernel(const GdJob *__restrict__ jobs,
-                                                         const GdBlock *__restrict__ blocks,
-                                                         int mode, int launch_step,
-                                                         const int *__restrict__ step_base,
-                                                         int *__restrict__ wq) {
+// Sum of n values at stride `st` (floats) with eight loads in flight; the
+// combination order depends only on n, so results are reproducible.
+__device__ __forceinline__ float gd_sum_strided(const float *__restrict__ p, int n, size_t st) {
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f, a5 = 0.0f, a6 = 0.0f, a7 = 0.0f;
+  int q = 0;
+  for (; q + 8 <= n; q += 8) {
+    const float *b = p + (size_t)q * st;
+    a0 += b[0];
+    a1 += b[st];
+    a2 += b[2 * st];
+    a3 += b[3 * st];
+    a4 += b[4 * st];
+    a5 += b[5 * st];
+    a6 += b[6 * st];
+    a7 += b[7 * st];
+  }
+  for (; q < n; ++q) a0 += p[(size_t)q * st];
+  return ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
+}
+
+// ---------------------------------------------------------------------------
+// Finish kernel: 1024 threads = 16 waves per 64-replica block; lanes are
+// replicas, waves split the partial slots, atoms and MCS rows.
+#define GD_FW 16
+__global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict__ jobs,
+                                                          const GdBlock *__restrict__ blocks,
+                                                          int mode, int launch_step,
+                                                          const int *__restrict__ step_base,
+                                                          int *__restrict__ wq) {
   if (wq && blockIdx.x == 0 && threadIdx.x == 0) *wq = 0;  // next pair launch's queue
   if (step_base) launch_step += *step_base;  // graph replays: base step in device memory
-  __shared__ float red[4][8][64];
+  __shared__ float red[GD_FW][8][64];
   __shared__ float xf[12][64];
   const GdBlock bk = blocks[blockIdx.x];
   const GdJob &J = jobs[bk.job];
@@ -370,29 +406,25 @@ __global__ __launch_bounds__(256) void gd_finish_kernel(const GdJob *__restrict_
   const int nS = J.nL * J.nP;
   const int step = launch_step - J.step0;  // this job's MC step
 
-  // (1) slot reductions, waves interleaved over slots / atoms / rows
+  // (1) slot reductions: wave w takes the contiguous w-th share of the slots
+  // of each linear term, atoms l = w, w+16, ..., rows m = w, w+16, ...
   float acc[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
+  {
+    const int q0 = (int)(((long long)nS * wave) / GD_FW), q1 = (int)(((long long)nS * (wave + 1)) / GD_FW);
 #pragma unroll
-  for (int k = 0; k < GD_NLIN; ++k) {
-    const float *src = J.part_lin + (size_t)k * nS * Rp + r;
-    float s = 0.0f;
-    for (int q = wave; q < nS; q += 4) s += src[(size_t)q * Rp];
-    acc[k] = s;
+    for (int k = 0; k < GD_NLIN; ++k)
+      acc[k] = gd_sum_strided(J.part_lin + ((size_t)k * nS + q0) * Rp + r, q1 - q0, (size_t)Rp);
   }
-  for (int l = wave; l < J.L; l += 4) {  // hpc per atom (energy_prt_v1.C:132-133)
-    const float *src = J.part_hpc + (size_t)l * J.nP * Rp + r;
-    float h = 0.0f;
-    for (int b = 0; b < J.nP; ++b) h += src[(size_t)b * Rp];
+  for (int l = wave; l < J.L; l += GD_FW) {  // hpc per atom (energy_prt_v1.C:132-133)
+    const float h = gd_sum_strided(J.part_hpc + (size_t)l * J.nP * Rp + r, J.nP, (size_t)Rp);
     const GdAtom at = J.atoms[l];
     const float t = (h - at.hpl0) / at.hpl1;
     acc[6] += 0.5f * t * t - at.hpl2;
   }
-  for (int m = wave; m < J.M; m += 4) {  // MCS rows (energy_mcs_v1.C:5-29)
-    const float *src = J.part_mcs + (size_t)m * J.nL * Rp + r;
-    float d = 0.0f;
-    for (int a = 0; a < J.nL; ++a) d += src[(size_t)a * Rp];
+  for (int m = wave; m < J.M; m += GD_FW) {  // MCS rows (energy_mcs_v1.C:5-29)
+    const float d = gd_sum_strided(J.part_mcs + (size_t)m * J.nL * Rp + r, J.nL, (size_t)Rp);
     const float2 row = J.mcsrow[m];
     acc[7] += row.x * sqrtf(d / row.y);
   }
@@ -403,7 +435,12 @@ __global__ __launch_bounds__(256) void gd_finish_kernel(const GdJob *__restrict_
   if (wave == 0) {
     float s[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) s[k] = red[0][k][lane] + red[1][k][lane] + red[2][k][lane] + red[3][k][lane];
+    for (int k = 0; k < 8; ++k) {
+      float v = 0.0f;
+#pragma unroll
+      for (int w = 0; w < GD_FW; ++w) v += red[w][k][lane];
+      s[k] = v;
+    }
     float m[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) m[i] = J.prop[(size_t)i * Rp + r];
@@ -502,7 +539,7 @@ __global__ __launch_bounds__(256) void gd_finish_kernel(const GdJob *__restrict_
   t[0] = xf[9][lane];
   t[1] = xf[10][lane];
   t[2] = xf[11][lane];
-  gd_transform_atoms(J, r, R, t, wave, 4);
+  gd_transform_atoms(J, r, R, t, wave, GD_FW);
 }
 
 // ---------------------------------------------------------------------------

@@ -215,6 +215,12 @@ class DeviceJob:
                             C.byref(st) if st is not None else None), "gd_dock")
         return out, st
 
+    def replicas(self):
+        """Current device replica state (gd_job_replicas)."""
+        out = ReplicaOut(self.n_rep)
+        check(lib().gd_job_replicas(self.h, C.byref(out.c)), "gd_job_replicas")
+        return out
+
     def records(self, cap):
         rec = np.zeros((self.n_rep, cap, 17), np.float32)
         cnt = np.zeros(self.n_rep, np.int32)

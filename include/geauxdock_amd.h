@@ -10,7 +10,9 @@
  *   gd_dock              <- Dock (GPU backend)          src/backend_gpu/backend_gpu.h:18-23
  *   gd_energy_eval       <- the s2max==1 initial-energy launch of MonteCarlo_d
  *                           src/backend_gpu/backend_gpu.cu:87-91 (kernel_cuda_l1_montecarlo.cu:29)
- *   gd_rng_seed          <- InitCurand                  src/backend_gpu/backend_gpu.h:16
+ *   gd_mc_param.seed/job_id <- InitCurand (per-replica cuRAND states are
+ *                           replaced by a counter-based Philox stream keyed by
+ *                           seed; no device state)   src/backend_gpu/backend_gpu.h:4
  *   gd_set_temperature   <- SetTemperature              src/frontend/util.C:325-363
  *   gd_load_complex      <- server1.C job assembly      src/frontend/server1.C:54-217
  *   gd_batch_dock        <- (no counterpart: many complexes in one launch sequence)
@@ -172,6 +174,11 @@ gd_status gd_energy_eval(gd_job *job, int n_pose, const int *conf, const float *
 /* Plain one-job docking: initial-energy pass, then steps_total MC steps with
  * temperature exchange, replica state resident on the device. */
 gd_status gd_dock(gd_job *job, const gd_mc_param *mc, gd_replica_out *out, gd_stats *stats);
+
+/* Current replica state of the job, read back from the device (the state
+ * stays resident; the reference left the host copy stale after Dock,
+ * backend_gpu.cu:64-133).  Same content as gd_dock's `out`. */
+gd_status gd_job_replicas(gd_job *job, gd_replica_out *out);
 
 /* Accept records of the last dump segment: rec[n_rep][cap] entries of
  * (step, movematrix[6], energy[10]) packed as 17 floats; count[n_rep]. */

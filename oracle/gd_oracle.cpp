@@ -554,6 +554,14 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
   return GD_OK;
 }
 
+// The CPU backend's entry point Dock(Complex*, Record*)
+// (src/backend_cpu_mic/backend_cpu_mic.h:4) in C-ABI form: whole replica
+// range, every host thread, the Philox stream of mc->seed / mc->job_id.
+int gd_cpu_dock(const gd_job_desc *d, const gd_mc_param *mc, gd_replica_out *out) {
+  if (!d) return GD_ERR_ARG;
+  return or_dock(d, mc, 0, 0, d->n_lig * d->n_prt * d->n_tmp, out, nullptr, nullptr);
+}
+
 // Smallest float x with expf(x) > 0: the forced initial accept of the
 // reference fails below it (expf underflows to zero).
 float or_exp_underflow_threshold(void) {

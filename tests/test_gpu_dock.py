@@ -104,6 +104,10 @@ def test_dock_is_deterministic(small_job):
     a, _ = dev.dock(mc)
     b, _ = dev.dock(mc)
     assert np.array_equal(a.energy, b.energy) and np.array_equal(a.movematrix, b.movematrix)
+    # the replica state stays on the device and reads back unchanged
+    c = dev.replicas()
+    for f in ("movematrix", "energy", "idx_tmp", "n_accept", "n_exchange", "best_energy", "is_accept"):
+        assert np.array_equal(getattr(b, f), getattr(c, f)), f
 
 
 def _batch(jobs, mcs, stats=False):

@@ -111,3 +111,19 @@ def test_tiling_invariance(ene, oracle):
         if ref is not None:
             assert np.abs(t - ref).max() < 1e-4
         ref = t
+
+
+@pytest.mark.parametrize("el0,el1", [(2.0, 0.1), (1.5, 0.1)])
+def test_electrostatics_both_forms(ene, oracle, el0, el1):
+    """el1 = 0.1 puts every pair under 10 A on the polynomial branch.  With the
+    reference el0 = 2 the kernel evaluates min(poly, 1/s); el0 = 1.5 breaks the
+    crossing at s = 1 and selects the branching form."""
+    import ctypes as C
+    from geauxdock_amd import abi
+    e = abi.gd_enepara()
+    C.memmove(C.byref(e), C.byref(ene), C.sizeof(abi.gd_enepara))
+    e.el0, e.el1 = el0, el1
+    job = g.synth_complex(1500, 24, 3, 2, 2, seed=17, ene=e)
+    dev = g.DeviceJob(job)
+    conf, pose = random_poses(job, 128, seed=23, trans=1.5)
+    _parity(job, dev, oracle, conf, pose, "el0=%g el1=%g" % (el0, el1))

@@ -318,3 +318,22 @@ def test_exchange_ladder_is_permutation(oracle):
     assert sorted(tmp[:]) == list(range(nt))
     # strongly ordered energies: after enough rounds the lowest energy sits at T0
     assert tmp[nt - 1] == 0
+
+
+def test_cpu_dock_entry_point(golden_job, oracle):
+    """gd_cpu_dock (the CPU backend's Dock in C-ABI form): the initial pass of the
+    golden complex reproduces the published line; a short run equals or_dock."""
+    from geauxdock_amd import abi
+    from geauxdock_amd.host import ReplicaOut
+    L = oracle.lib
+    L.gd_cpu_dock.argtypes = [C.c_void_p, C.POINTER(abi.gd_mc_param), C.POINTER(abi.gd_replica_out)]
+    mc = g.make_mc(0, move_mode="reference")
+    out = ReplicaOut(golden_job.n_rep)
+    assert L.gd_cpu_dock(C.byref(golden_job.desc), C.byref(mc), C.byref(out.c)) == 0
+    assert np.all(out.is_accept == 1)
+    assert " ".join("%.4f" % x for x in out.energy[0]) == " ".join("%.4f" % x for x in GOLDEN)
+    mc = g.make_mc(5, move_mode="gaussian", seed=9)
+    out = ReplicaOut(golden_job.n_rep)
+    assert L.gd_cpu_dock(C.byref(golden_job.desc), C.byref(mc), C.byref(out.c)) == 0
+    ref, _, _ = oracle.dock(golden_job, mc, threads=2)
+    assert np.array_equal(out.energy, ref.energy) and np.array_equal(out.n_accept, ref.n_accept)
