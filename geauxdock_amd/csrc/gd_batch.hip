@@ -22,7 +22,7 @@
 
 #include "gd_internal.h"
 
-template <int LC, int EMIN>
+template <int LC>
 __global__ void gd_pair_kernel_wq(const GdJob *, const GdItem *, int, int *);
 __global__ void gd_transform_kernel(const GdJob *, const GdBlock *, int *);
 __global__ void gd_finish_kernel(const GdJob *, const GdBlock *, int, int, const int *, int *);
@@ -32,17 +32,20 @@ using namespace gdr;
 
 namespace {
 
-hipError_t launch_pair_wq(int lc, int emin, const GdJob *jobs, const GdItem *items, int n_items,
-                          int *wq, int grid, hipStream_t s) {
+hipError_t launch_pair_wq(int lc, const GdJob *jobs, const GdItem *items, int n_items, int *wq,
+                          int grid, hipStream_t s) {
   if (n_items <= 0) return hipSuccess;
   const dim3 g(grid), b(256);
-  switch (lc * 2 + (emin ? 1 : 0)) {
-    case 8: hipLaunchKernelGGL((gd_pair_kernel_wq<4, 0>), g, b, 0, s, jobs, items, n_items, wq); break;
-    case 9: hipLaunchKernelGGL((gd_pair_kernel_wq<4, 1>), g, b, 0, s, jobs, items, n_items, wq); break;
-    case 16: hipLaunchKernelGGL((gd_pair_kernel_wq<8, 0>), g, b, 0, s, jobs, items, n_items, wq); break;
-    case 17: hipLaunchKernelGGL((gd_pair_kernel_wq<8, 1>), g, b, 0, s, jobs, items, n_items, wq); break;
-    case 33: hipLaunchKernelGGL((gd_pair_kernel_wq<16, 1>), g, b, 0, s, jobs, items, n_items, wq); break;
-    default: hipLaunchKernelGGL((gd_pair_kernel_wq<16, 0>), g, b, 0, s, jobs, items, n_items, wq); break;
+  switch (lc) {
+    case 4:
+      hipLaunchKernelGGL(gd_pair_kernel_wq<4>, g, b, 0, s, jobs, items, n_items, wq);
+      break;
+    case 8:
+      hipLaunchKernelGGL(gd_pair_kernel_wq<8>, g, b, 0, s, jobs, items, n_items, wq);
+      break;
+    default:
+      hipLaunchKernelGGL(gd_pair_kernel_wq<16>, g, b, 0, s, jobs, items, n_items, wq);
+      break;
   }
   return hipGetLastError();
 }
@@ -54,7 +57,6 @@ struct BatchLists {
   GdBlock *d_blocks = nullptr;
   int *d_wq = nullptr;       // work-queue counter
   int *d_base = nullptr;     // base launch step for graph replays
-  int emin = 1;              // every job evaluates electrostatics as a min
   GdExch *d_ex = nullptr;    // exchange entries (scratch, per launch)
   int n_items = 0, n_blocks = 0, n_jobs = 0, ex_cap = 0;
   void *mem = nullptr;
@@ -69,8 +71,6 @@ struct BatchLists {
     for (int k = 0; k < (int)which.size(); ++k) {
       gd_job *j = jobs[which[k]];
       js.push_back(j->h);
-      if (k == 0) emin = 1;
-      emin &= j->ele_min;
       // job-local items in protein-tile-major order: waves that start together
       // share the same protein tile (scalar-cache and L2 reuse)
       for (const GdItem &it : j->items) items.push_back({k, it.rb, it.a, it.b});
@@ -119,7 +119,7 @@ int pair_grid(int waves_per_cu) {
 gd_status batch_step(const BatchLists &B, int lc, int grid, int launch_step, const int *base,
                      const std::vector<GdExch> *ex, hipStream_t s, KTimer *tp) {
   if (tp && tp->begin(s) != GD_OK) return GD_ERR_HIP;
-  hipError_t e = launch_pair_wq(lc, B.emin, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s);
+  hipError_t e = launch_pair_wq(lc, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s);
   if (e != hipSuccess) return hip_fail(e, "pair kernel (queue) launch");
   if (tp && tp->end(s) != GD_OK) return GD_ERR_HIP;
   hipLaunchKernelGGL(gd_finish_kernel, dim3(B.n_blocks), dim3(1024), 0, s, B.d_jobs, B.d_blocks,
@@ -139,7 +139,7 @@ gd_status batch_init(const BatchLists &B, int lc, int grid, hipStream_t s) {
                      B.d_wq);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "transform kernel launch");
-  if ((e = launch_pair_wq(lc, B.emin, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s)) != hipSuccess)
+  if ((e = launch_pair_wq(lc, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s)) != hipSuccess)
     return hip_fail(e, "pair kernel (queue) launch");
   hipLaunchKernelGGL(gd_finish_kernel, dim3(B.n_blocks), dim3(1024), 0, s, B.d_jobs, B.d_blocks,
                      GD_MODE_INIT, 0, (const int *)nullptr, B.d_wq);

@@ -37,8 +37,6 @@ struct GdAtom {           // per ligand atom, type-sorted order
   int kde_b, kde_e;       // KDE anchors of the atom's type [b, e)
   float kde_inv_n;        // 1 / (e - b), 0 if empty
   float hpl0, hpl1, hpl2; // hydrophobic restraint of the atom's type
-  unsigned hmask;         // bit t set: H-bond term active against point type t
-  int pad_[3];
 };
 
 struct GdJob {
@@ -54,7 +52,7 @@ struct GdJob {
   const float4 *pcoord;   // [n_prt][P]
   const float4 *pattr;    // [P] (ele, hpp, flags-as-int, 0)
   const float *ptab;      // [n_types][GD_PTAB_PT][GD_PTAB_W]: p1a, -p2a, PMF exponent B and C,
-                          // pmf1, psp r^2 threshold, H-bond c and -hdb0 c
+                          // pmf1, psp r^2 threshold (sign bit: H-bond active), H-bond c, -hdb0 c
   const float *pspv;      // [n_types][P] psp value of class-2 points
   const float4 *kde;      // [K] sorted by type
   const float4 *mcs;      // [M][L] (x, y, z, valid)

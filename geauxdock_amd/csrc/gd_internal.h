@@ -23,7 +23,6 @@ struct gd_job {
   GdItem *d_items = nullptr;
   GdBlock *d_blocks = nullptr;
   int n_items = 0, n_blocks = 0;
-  int ele_min = 0;           // electrostatics as min(poly, 1/r) (checked at pack time)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -39,9 +38,7 @@ gd_status sync_job_struct(gd_job *j);
 gd_status reset_replicas(gd_job *j);
 void set_mc(GdJob &g, const gd_mc_param *mc);
 gd_status read_out(gd_job *j, gd_replica_out *out);
-gd_status launch_pair(int lc, int emin, const GdJob *d_jobs, const GdItem *d_items, int n_items,
-                      hipStream_t s);
-bool ele_min_ok(float el0, float el1, float a1, float b1);
+gd_status launch_pair(int lc, const GdJob *d_jobs, const GdItem *d_items, int n_items, hipStream_t s);
 gd_status launch_finish(const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks, int mode,
                         int step, hipStream_t s);
 gd_status launch_transform(const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks, hipStream_t s);

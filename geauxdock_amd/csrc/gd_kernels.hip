@@ -42,6 +42,8 @@ __device__ __forceinline__ float gd_exp2(float x) { return __builtin_amdgcn_exp2
 #define GD_HPB (4.5f / 81.0f / 81.0f)
 #define GD_HPC (2.5f / 81.0f / 81.0f / 81.0f)
 #define GD_HPD (0.5f / 81.0f / 81.0f / 81.0f / 81.0f)
+// PMF exponents below this give exp2(x) < 2^-25: 1 + exp2(x) rounds to 1
+#define GD_PMF_FAR (-25.0f)
 // 1 / sqrt(log2(e) / 2) = sqrt(2 ln 2): H-bond sums carry the factor sqrt(log2(e) / 2)
 #define GD_HDB_RC 1.17741002251547469f
 
@@ -74,7 +76,7 @@ __device__ __forceinline__ const GD_AS_GLOBAL T *gd_g(const T *p) {
 
 // ---------------------------------------------------------------------------
 // Pair kernel.  LC ligand atoms per wave, all LC coordinates in registers.
-template <int LC, int EMIN>
+template <int LC>
 __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, const GdItem it,
                                              const int lane) {
   const GD_AS_CONST GdJob &J = gd_c(jobs)[it.job];
@@ -93,7 +95,6 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   // ligand chunk: coordinates of this lane's replica, per-atom constants (uniform)
   float lx[LC], ly[LC], lz[LC], ele[LC], hpc[LC];
   int tid[LC];
-  unsigned hmask[LC];
 #pragma unroll
   for (int j = 0; j < LC; ++j) {
     const int l = l0 + min(j, nl - 1);
@@ -101,7 +102,6 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
     ly[j] = coords[(size_t)(3 * l + 1) * Rp + r];
     lz[j] = coords[(size_t)(3 * l + 2) * Rp + r];
     tid[j] = atoms[l].tidx * GD_PTAB_PT;
-    hmask[j] = atoms[l].hmask;
     ele[j] = 0.0f;
     hpc[j] = 0.0f;
   }
@@ -131,21 +131,28 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
         const GD_AS_CONST float *__restrict__ prm = ptab + (size_t)(tid[j] + pt) * GD_PTAB_W;
         const float p1a = prm[0], np2a = prm[1], pB = prm[2], pC = prm[3];
         const float pmf1 = prm[4], pthr = prm[5], hdb1c = prm[6], nhdb0c = prm[7];
+        // the sign bit of the psp threshold flags an active H-bond pair; the asm
+        // keeps the test on the scalar unit (as a float compare it became a VALU op)
+        int hflag;
+        asm("" : "=s"(hflag) : "0"(__float_as_int(pthr)));
         const float dx = lx[j] - c.x, dy = ly[j] - c.y, dz = lz[j] - c.z;
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         const float ri = gd_rsq(r2);
         const float rr = r2 * ri;
         // hydrophobic contact (energy_prt_v1.C:44-50):
-        // hpp (1 - r^2 (a - b r^2 + c r^4 - d r^6)) for r^2 <= 81
-        // (the empty volatile asm keeps this a wave-uniform branch; without it
-        // the compiler evaluates the term for every point and selects)
+        // hpp (1 - r^2 (a - b r^2 + c r^4 - d r^6)) for r^2 <= 81, on the lanes
+        // within 9 A only (exec-masked; skipped when no lane is).  The empty
+        // volatile asm statements keep both branches: without them the compiler
+        // evaluates the compare, or the whole term, for every pair
         if (fl & GD_PF_CDC) {
           asm volatile("");
-          float w = __builtin_fmaf(-GD_HPD, r2, GD_HPC);
-          w = __builtin_fmaf(w, r2, -GD_HPB);
-          w = __builtin_fmaf(w, r2, GD_HPA);
-          const float v = __builtin_fmaf(-r2, w, 1.0f);
-          hpc[j] = __builtin_fmaf(phpp, (r2 <= 81.0f) ? v : 0.0f, hpc[j]);
+          if (r2 <= 81.0f) {
+            asm volatile("");
+            float w = __builtin_fmaf(-GD_HPD, r2, GD_HPC);
+            w = __builtin_fmaf(w, r2, -GD_HPB);
+            w = __builtin_fmaf(w, r2, GD_HPA);
+            hpc[j] = __builtin_fmaf(phpp, __builtin_fmaf(-r2, w, 1.0f), hpc[j]);
+          }
         }
         // soft Lennard-Jones (:61-64): (p1 - p2) / (p1 lj0 (1 + lj1 r^2) + 1) with
         // p1 = p1a / r^9, p2 = p2a / r^6, multiplied through by r^9:
@@ -157,31 +164,40 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
           const float den = __builtin_fmaf(p1a, __builtin_fmaf(lj01, r2, lj0), r9);
           evdw = __builtin_fmaf(num, gd_rcp(den), evdw);
         }
-        // distance-dependent electrostatics (:72-78), charge applied per atom.
-        // EMIN: the host checked that s·poly(s) - 1 is <= 0 below s = 1 and >= 0
-        // above it (true of the reference parameters, where it is (s-1)^3 (s+1)),
-        // so the branch on s < 1 is the smaller of the two forms
-        {
-          const float poly = __builtin_fmaf(__builtin_fmaf(e3, rr, e2), r2, e0);
-          const float g = EMIN ? fminf(poly, ri) : ((rr < inv_el1) ? poly : ri);
-          ele[j] = __builtin_fmaf(pele, g, ele[j]);
-        }
         // contact PMF (:88-91): exponent (6 - r/2)(r - pmf0) log2(e)
         // = -log2(e)/2 r^2 + pB r + pC
-        {
-          const float arg = __builtin_fmaf(pB, rr, __builtin_fmaf(pmf_a, r2, pC));
+        const float arg = __builtin_fmaf(pB, rr, __builtin_fmaf(pmf_a, r2, pC));
+        // Far field, decided for the whole wave: below arg = -25, exp2(arg) <
+        // 2^-25 and 1 + exp2(arg) rounds to 1, so the PMF term is exactly pmf1;
+        // at s = el1 r >= 1 the electrostatic factor is 1/s (scaled: 1/r).
+        const bool ele_near = rr < inv_el1;
+        const bool near = (arg >= GD_PMF_FAR) | ele_near;
+        float g = ri;
+        if (__ballot(near) != 0) {
+          asm volatile("");
+          // distance-dependent electrostatics (:72-78), charge applied per atom:
+          // el1 g = poly(r) for s = el1 r < 1, else 1/r
+          const float poly = __builtin_fmaf(__builtin_fmaf(e3, rr, e2), r2, e0);
+          g = ele_near ? poly : ri;
           epmf = __builtin_fmaf(pmf1, gd_rcp(1.0f + gd_exp2(arg)), epmf);
+        } else {
+          epmf += pmf1;
         }
+        ele[j] = __builtin_fmaf(pele, g, ele[j]);
         // pocket-specific potential (:100-107): r <= pmf0 decided on r^2
-        // against the exact float threshold of sqrtf(r^2) <= pmf0
+        // against the exact float threshold of sqrtf(r^2) <= pmf0; the value is
+        // loaded only when some lane is in contact (exec-masked branch)
         if (fl & GD_PF_CLS2) {
-          const float v = pspv[(size_t)(tid[j] / GD_PTAB_PT) * P + p];
-          epsp += (r2 <= pthr) ? v : 0.0f;
+          asm volatile("");
+          if (r2 <= fabsf(pthr)) {
+            asm volatile("");
+            epsp += pspv[(size_t)(tid[j] / GD_PTAB_PT) * P + p];
+          }
         }
         // hydrogen bond (:116-121) as c exp2(-(r c - hdb0 c)^2), c = hdb1 sqrt(log2(e)/2),
-        // summed without the 1/sqrt(log2(e)/2) factor; pairs with hdb0 <= 0.1 are
-        // off in the atom's point-type mask and skip the term on a wave-uniform branch
-        if ((hmask[j] >> pt) & 1u) {
+        // summed without the 1/sqrt(log2(e)/2) factor; pairs with hdb0 <= 0.1 skip
+        // the term on a wave-uniform branch
+        if (hflag < 0) {
           asm volatile("");
           const float h = __builtin_fmaf(rr, hdb1c, nhdb0c);
           ehdb = __builtin_fmaf(hdb1c, gd_exp2(-(h * h)), ehdb);
@@ -254,13 +270,13 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
 }
 
 // One wave per work item (grid = items / 4 workgroups of 4 waves).
-template <int LC, int EMIN>
+template <int LC>
 __global__ __launch_bounds__(256) void gd_pair_kernel(const GdJob *__restrict__ jobs,
                                                        const GdItem *__restrict__ items,
                                                        int n_items) {
   const int iid = gd_uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
   if (iid >= n_items) return;
-  gd_pair_item<LC, EMIN>(jobs, gd_c(items)[iid], threadIdx.x & 63);
+  gd_pair_item<LC>(jobs, gd_c(items)[iid], threadIdx.x & 63);
 }
 
 // Persistent form: a fixed grid sized to the machine; every wave pulls item
@@ -268,7 +284,7 @@ __global__ __launch_bounds__(256) void gd_pair_kernel(const GdJob *__restrict__ 
 // items of uneven cost (complexes of different sizes in one launch).  The
 // counter is zeroed by the kernel that runs before each launch (finish or
 // transform kernel); every wave exits once the counter passes n_items.
-template <int LC, int EMIN>
+template <int LC>
 __global__ __launch_bounds__(256) void gd_pair_kernel_wq(const GdJob *__restrict__ jobs,
                                                           const GdItem *__restrict__ items,
                                                           int n_items, int *__restrict__ wq) {
@@ -278,19 +294,16 @@ __global__ __launch_bounds__(256) void gd_pair_kernel_wq(const GdJob *__restrict
     if (lane == 0) iid = atomicAdd(wq, 1);
     iid = gd_uniform(__shfl(iid, 0));
     if (iid >= n_items) break;
-    gd_pair_item<LC, EMIN>(jobs, gd_c(items)[iid], lane);
+    gd_pair_item<LC>(jobs, gd_c(items)[iid], lane);
   }
 }
 
-#define GD_PAIR_INST(LC, EM)                                                              \
-  template __global__ void gd_pair_kernel<LC, EM>(const GdJob *, const GdItem *, int);    \
-  template __global__ void gd_pair_kernel_wq<LC, EM>(const GdJob *, const GdItem *, int, int *);
-GD_PAIR_INST(4, 0)
-GD_PAIR_INST(8, 0)
-GD_PAIR_INST(16, 0)
-GD_PAIR_INST(4, 1)
-GD_PAIR_INST(8, 1)
-GD_PAIR_INST(16, 1)
+template __global__ void gd_pair_kernel<4>(const GdJob *, const GdItem *, int);
+template __global__ void gd_pair_kernel<8>(const GdJob *, const GdItem *, int);
+template __global__ void gd_pair_kernel<16>(const GdJob *, const GdItem *, int);
+template __global__ void gd_pair_kernel_wq<4>(const GdJob *, const GdItem *, int, int *);
+template __global__ void gd_pair_kernel_wq<8>(const GdJob *, const GdItem *, int, int *);
+template __global__ void gd_pair_kernel_wq<16>(const GdJob *, const GdItem *, int, int *);
 
 // ---------------------------------------------------------------------------
 // Ligand transform of one replica's proposal (kernel_cpp_montecarlo_soa.C:

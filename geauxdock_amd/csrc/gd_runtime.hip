@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -26,7 +27,7 @@
 #include "gd_internal.h"
 
 // kernels (gd_kernels.hip)
-template <int LC, int EMIN>
+template <int LC>
 __global__ void gd_pair_kernel(const GdJob *, const GdItem *, int);
 __global__ void gd_transform_kernel(const GdJob *, const GdBlock *, int *);
 __global__ void gd_finish_kernel(const GdJob *, const GdBlock *, int, int, const int *, int *);
@@ -183,25 +184,6 @@ gd_status apply_tiling(gd_job *j, int lc, int ptile) {
 
 // ---------------------------------------------------------------------------
 // Pack a complex into the device layout.
-// The electrostatic factor is g(s) = el0 + a1 s^2 + b1 s^3 for s = el1 r < 1 and
-// 1/s otherwise (energy_prt_v1.C:72-78).  When f(s) = s g_poly(s) - 1 is <= 0
-// on (0, 1) and >= 0 on [1, inf), the branch equals min(g_poly, 1/s) for every
-// s > 0, which the pair kernel evaluates in one instruction instead of two.
-// The reference parameters (el0 = 2, a1 = -2, b1 = 1) give f = (s-1)^3 (s+1).
-bool ele_min_ok(float el0, float el1, float a1, float b1) {
-  if (!(el1 > 0.0f) || !(b1 > 0.0f)) return false;
-  const double c0 = el0, c2 = a1, c3 = b1;
-  auto f = [&](double x) { return ((c3 * x + c2) * x * x + c0) * x - 1.0; };
-  const double tol = 1e-7;
-  for (int i = 1; i < 200000; ++i)
-    if (f(i / 200000.0) > tol) return false;
-  // beyond s_hi the quartic term dominates every other: f > 0
-  const double s_hi = std::max(2.0, 2.0 * (std::fabs(c2) + std::fabs(c0) + 1.0) / c3);
-  for (int i = 0; i <= 200000; ++i)
-    if (f(1.0 + (s_hi - 1.0) * i / 200000.0) < -tol) return false;
-  return true;
-}
-
 static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
   const int L = d->lig_natom, P = d->prt_npoint, K = d->kde_npoint, M = d->mcs_nrow;
   const int nl = d->n_lig, np = d->n_prt, nt = d->n_tmp;
@@ -253,10 +235,13 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       o[2] = (float)(log2e * (6.0 + 0.5 * pmf0));
       o[3] = (float)(-6.0 * log2e * pmf0);
       o[4] = e->pmf1[lt][pt];
-      o[5] = sqrt_threshold(e->pmf0[lt][pt]);
+      // psp cutoff on r^2 (compared against |o[5]|; NaN: never); the sign bit
+      // marks the pairs whose H-bond term is on
+      const float thr = sqrt_threshold(e->pmf0[lt][pt]);
+      const bool hb = e->hdb0[lt][pt] > 0.1f;
+      o[5] = std::copysign(thr >= 0.0f ? thr : std::numeric_limits<float>::quiet_NaN(), hb ? -1.0f : 1.0f);
       // H-bond: hdb1 exp(-(r - hdb0)^2 hdb1^2 / 2) = (c / k) exp2(-(r c - hdb0 c)^2),
       // k = sqrt(log2(e)/2), c = hdb1 k; rows with hdb0 <= 0.1 are off (c = 0)
-      const bool hb = e->hdb0[lt][pt] > 0.1f;
       const double c = (double)e->hdb1[lt][pt] * hdb_c;
       o[6] = hb ? (float)c : 0.0f;
       o[7] = hb ? (float)(-(double)e->hdb0[lt][pt] * c) : 0.0f;
@@ -302,10 +287,6 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
     a.hpl0 = e->hpl0[lt];
     a.hpl1 = e->hpl1[lt];
     a.hpl2 = e->hpl2[lt];
-    a.hmask = 0;
-    for (int pt = 0; pt < GD_NPT; ++pt)
-      if (e->hdb0[lt][pt] > 0.1f) a.hmask |= 1u << pt;
-    a.pad_[0] = a.pad_[1] = a.pad_[2] = 0;
   }
   std::vector<float4> mcs((size_t)std::max(1, M) * L);
   std::vector<float2> mcsrow(std::max(1, M));
@@ -334,7 +315,6 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
   // scalars
   g.lj0 = e->lj0, g.lj1 = e->lj1, g.el0 = e->el0, g.el1 = e->el1, g.a1 = e->a1, g.b1 = e->b1;
   g.inv_el1 = 1.0f / e->el1;
-  j->ele_min = ele_min_ok(e->el0, e->el1, e->a1, e->b1) ? 1 : 0;
   g.kde2l = e->kde2 * 1.44269504088896340736f;
   g.kde3_inv = 1.0f / e->kde3;
   g.inv_L = 1.0f / (float)L;
@@ -481,18 +461,20 @@ gd_status sync_job_struct(gd_job *j) {
 }
 
 // ---- launches ---------------------------------------------------------------
-gd_status launch_pair(int lc, int emin, const GdJob *d_jobs, const GdItem *d_items, int n_items,
+gd_status launch_pair(int lc, const GdJob *d_jobs, const GdItem *d_items, int n_items,
                       hipStream_t s) {
   if (n_items <= 0) return GD_OK;
   const dim3 grid((n_items + 3) / 4), block(256);
-  const int v = lc * 2 + (emin ? 1 : 0);
-  switch (v) {
-    case 8: hipLaunchKernelGGL((gd_pair_kernel<4, 0>), grid, block, 0, s, d_jobs, d_items, n_items); break;
-    case 9: hipLaunchKernelGGL((gd_pair_kernel<4, 1>), grid, block, 0, s, d_jobs, d_items, n_items); break;
-    case 16: hipLaunchKernelGGL((gd_pair_kernel<8, 0>), grid, block, 0, s, d_jobs, d_items, n_items); break;
-    case 17: hipLaunchKernelGGL((gd_pair_kernel<8, 1>), grid, block, 0, s, d_jobs, d_items, n_items); break;
-    case 32: hipLaunchKernelGGL((gd_pair_kernel<16, 0>), grid, block, 0, s, d_jobs, d_items, n_items); break;
-    case 33: hipLaunchKernelGGL((gd_pair_kernel<16, 1>), grid, block, 0, s, d_jobs, d_items, n_items); break;
+  switch (lc) {
+    case 4:
+      hipLaunchKernelGGL(gd_pair_kernel<4>, grid, block, 0, s, d_jobs, d_items, n_items);
+      break;
+    case 8:
+      hipLaunchKernelGGL(gd_pair_kernel<8>, grid, block, 0, s, d_jobs, d_items, n_items);
+      break;
+    case 16:
+      hipLaunchKernelGGL(gd_pair_kernel<16>, grid, block, 0, s, d_jobs, d_items, n_items);
+      break;
     default:
       return GD_ERR_ARG;
   }
@@ -757,7 +739,7 @@ gd_status gd_energy_eval(gd_job *j, int n, const int *conf, const float *pose, f
       break;
     }
     if ((st = launch_transform(j->d_job, j->d_blocks, j->n_blocks, s)) != GD_OK) break;
-    if ((st = launch_pair(g.lc, j->ele_min, j->d_job, j->d_items, j->n_items, s)) != GD_OK) break;
+    if ((st = launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s)) != GD_OK) break;
     if ((st = launch_finish(j->d_job, j->d_blocks, j->n_blocks, GD_MODE_EVAL, 0, s)) != GD_OK) break;
     if ((e = hipMemcpyAsync(ot.data(), g.ev_terms, (size_t)10 * Rp * sizeof(float), hipMemcpyDeviceToHost, s)) != hipSuccess ||
         (e = hipMemcpyAsync(orw.data(), g.ev_raw, (size_t)9 * Rp * sizeof(float), hipMemcpyDeviceToHost, s)) != hipSuccess ||
@@ -839,7 +821,7 @@ gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stat
   const auto t0 = std::chrono::steady_clock::now();
   // initial energies (MonteCarlo_d with s2max = 1)
   if (st == GD_OK) st = launch_transform(j->d_job, j->d_blocks, j->n_blocks, s);
-  if (st == GD_OK) st = launch_pair(g.lc, j->ele_min, j->d_job, j->d_items, j->n_items, s);
+  if (st == GD_OK) st = launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s);
   if (st == GD_OK) st = launch_finish(j->d_job, j->d_blocks, j->n_blocks, GD_MODE_INIT, 0, s);
   for (int s1 = 0; s1 < mc->steps_total && st == GD_OK; s1 += spd) {
     if (mc->record) {
@@ -852,7 +834,7 @@ gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stat
     const int s_end = std::min(mc->steps_total, s1 + spd);
     for (int step = s1; step < s_end; ++step) {
       if (timing && (st = tp.begin(s)) != GD_OK) break;
-      if ((st = launch_pair(g.lc, j->ele_min, j->d_job, j->d_items, j->n_items, s)) != GD_OK) break;
+      if ((st = launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s)) != GD_OK) break;
       if (timing && (st = tp.end(s)) != GD_OK) break;
       if ((st = launch_finish(j->d_job, j->d_blocks, j->n_blocks, GD_MODE_STEP, step, s)) != GD_OK) break;
       if (d_ex && (step + 1) % spe == 0 && (step + 1) / spe <= n_rounds)

@@ -114,10 +114,10 @@ def test_tiling_invariance(ene, oracle):
 
 
 @pytest.mark.parametrize("el0,el1", [(2.0, 0.1), (1.5, 0.1)])
-def test_electrostatics_both_forms(ene, oracle, el0, el1):
-    """el1 = 0.1 puts every pair under 10 A on the polynomial branch.  With the
-    reference el0 = 2 the kernel evaluates min(poly, 1/s); el0 = 1.5 breaks the
-    crossing at s = 1 and selects the branching form."""
+def test_electrostatics_short_range(ene, oracle, el0, el1):
+    """el1 = 0.1 puts every pair under 10 A on the polynomial branch of the
+    electrostatic factor (and out of the far-field shortcut).  el0 = 2 is the
+    reference value, continuous at s = 1; el0 = 1.5 makes the branch jump there."""
     import ctypes as C
     from geauxdock_amd import abi
     e = abi.gd_enepara()
