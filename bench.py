@@ -173,6 +173,45 @@ def serve(rates, n_jobs, step_scale):
             "plain_note": "same requests, one job per launch sequence, back to back"}
 
 
+def _counter_means(run_dir, counter):
+    """Mean per-dispatch value of one counter, by kernel name, from a
+    rocprofv3 --pmc --output-format csv directory."""
+    import csv
+    import glob
+    vals = {}
+    for f in glob.glob(os.path.join(run_dir, "*counter_collection.csv")):
+        for r in csv.DictReader(open(f)):
+            if r["Counter_Name"] == counter:
+                vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in vals.items()}
+
+
+def traffic_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out_json):
+    """HBM bytes per pair-kernel launch from FETCH_SIZE / WRITE_SIZE runs (KiB per
+    dispatch), each scaled by the same counter's reading for a plain 256 MiB copy."""
+    n = 256 << 20
+    cf = _counter_means(copy_fetch_dir, "FETCH_SIZE")
+    cw = _counter_means(copy_write_dir, "WRITE_SIZE")
+    copy_k = [k for k in cf if "copyBuffer" in k or "elementwise" in k]
+    kf = max(copy_k, key=lambda k: cf[k])          # the copy: the largest reader
+    fetch_scale = n / (cf[kf] * 1024.0)
+    kw = kf if kf in cw else max(cw, key=lambda k: cw[k])
+    write_scale = n / (cw[kw] * 1024.0)
+    f = _counter_means(fetch_dir, "FETCH_SIZE")
+    w = _counter_means(write_dir, "WRITE_SIZE")
+    res = {"copy_bytes": n, "copy_kernel": kf, "fetch_scale": fetch_scale, "write_scale": write_scale,
+           "kernels": {}}
+    for k in f:
+        res["kernels"][k] = {"fetch_kib": f[k], "write_kib": w.get(k, 0.0),
+                             "hbm_bytes": f[k] * 1024.0 * fetch_scale + w.get(k, 0.0) * 1024.0 * write_scale}
+    pair = [k for k in res["kernels"] if "gd_pair_kernel" in k]
+    if pair:
+        res["hbm_bytes_per_launch"] = res["kernels"][pair[0]]["hbm_bytes"]
+        res["kernel"] = pair[0]
+    json.dump(res, open(out_json, "w"), indent=1)
+    return res
+
+
 def copy_calibration(n_bytes=256 << 20, reps=3):
     """A plain device-to-device copy of known size: the HBM counters
     (FETCH_SIZE, WRITE_SIZE) of its dispatches against n_bytes give the scale
@@ -197,9 +236,16 @@ def main():
     ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "copy"], default="c2",
                     help="c2 = the headline line; c3/c4 batched vs plain; c5 serving (extra lines); "
                          "copy = a plain 256 MiB device copy, the calibration for HBM counters")
+    ap.add_argument("--traffic", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="summarise rocprofv3 FETCH_SIZE/WRITE_SIZE runs into "
+                         "profiles/pair_kernel_c2_traffic.json (no GPU needed)")
     ap.add_argument("--jobs", type=int, default=32)
     ap.add_argument("--mc-steps", type=int, default=500)
     args = ap.parse_args()
+    if args.traffic:
+        print(json.dumps(traffic_summary(*args.traffic, os.path.join(ROOT, "profiles",
+                                                                      "pair_kernel_c2_traffic.json"))))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
               "(jobs and (ligand, protein) replica groups shard across GPUs without communication)" % args.gpus,

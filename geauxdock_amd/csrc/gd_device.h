@@ -19,7 +19,8 @@
 #define GD_PTAB_W 8       // floats per (ligand type, point type) entry
 #define GD_PTAB_PT 32     // point-type stride of the table (31 used)
 #define GD_REC_W 17       // floats per accept record: step, movematrix[6], energy[10]
-#define GD_NLIN 6         // linear partial sums: vdw ele pmf psp hdb kde
+#define GD_NLIN 6
+#define GD_GROUP 4        // protein tiles per work group (summed in LDS before writing)         // linear partial sums: vdw ele pmf psp hdb kde
 
 // point flag bits (pattr.z as int)
 #define GD_PF_TYPE 0x1f
@@ -42,20 +43,24 @@ struct GdAtom {           // per ligand atom, type-sorted order
 struct GdJob {
   // sizes and tiling
   int L, P, K, M;
+  int Ls;                 // atom stride of atoms / mcs: L padded to a multiple of 16 with
+                          // dummy atoms whose every term is exactly zero
   int n_lig, n_prt, n_tmp, R, Rp;
-  int nL, nP, ptile;      // ligand chunks, protein tiles, points per tile
+  int nL, nP, ptile;      // ligand chunks, protein tiles (a multiple of GD_GROUP), points per tile
+  int nG;                 // tile groups = nP / GD_GROUP: the partial-sum slots per chunk
   int lc;                 // ligand atoms per chunk (kernel template value)
   int n_types;            // rows of the parameter table
   // constant tables (device)
   const float4 *lig0;     // [n_lig][L] relative conformer coordinates
-  const GdAtom *atoms;    // [L]
+  const GdAtom *atoms;    // [Ls]
   const float4 *pcoord;   // [n_prt][P]
   const float4 *pattr;    // [P] (ele, hpp, flags-as-int, 0)
-  const float *ptab;      // [n_types][GD_PTAB_PT][GD_PTAB_W]: p1a, -p2a, PMF exponent B and C,
+  const float *ptab;      // [n_types + 1][GD_PTAB_PT][GD_PTAB_W] (last block: padding atoms):
+                          // p1a, -p2a, PMF exponent B and C,
                           // pmf1, psp r^2 threshold (sign bit: H-bond active), H-bond c, -hdb0 c
   const float *pspv;      // [n_types][P] psp value of class-2 points
   const float4 *kde;      // [K] sorted by type
-  const float4 *mcs;      // [M][L] (x, y, z, valid)
+  const float4 *mcs;      // [M][Ls] (x, y, z, valid)
   const float2 *mcsrow;   // [M] (tcc, n_valid)
   const float *minus_beta;// [n_tmp]
   // replica state (device, SoA over Rp)
@@ -68,8 +73,8 @@ struct GdJob {
   float *best;            // [Rp]
   int *own;               // [n_prt*n_lig][n_tmp] replica at each temperature
   float *coords;          // [L][3][Rp] transformed ligand of the current proposal
-  float *part_lin;        // [GD_NLIN][nL*nP][Rp]
-  float *part_hpc;        // [L][nP][Rp]
+  float *part_lin;        // [GD_NLIN][nL*nG][Rp]
+  float *part_hpc;        // [L][nG][Rp]
   float *part_mcs;        // [M][nL][Rp]
   float *rec;             // [Rp][rec_cap][GD_REC_W] accept records (optional)
   int *rec_n;             // [Rp]
@@ -92,11 +97,11 @@ struct GdJob {
   int *wq;                // work-queue counter of the persistent pair kernel
 };
 
-struct GdItem {           // one wave of the pair kernel
+struct GdItem {           // one 4-wave work group of the pair kernel
   int job;
   int rb;                 // first replica of the 64-lane block
   int a;                  // ligand chunk
-  int b;                  // protein tile
+  int b;                  // protein tile group: tiles GD_GROUP*b .. GD_GROUP*b + GD_GROUP-1
 };
 
 struct GdBlock {          // one workgroup of the finish / transform kernels

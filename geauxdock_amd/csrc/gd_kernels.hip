@@ -76,59 +76,56 @@ __device__ __forceinline__ const GD_AS_GLOBAL T *gd_g(const T *p) {
 
 // ---------------------------------------------------------------------------
 // Pair kernel.  LC ligand atoms per wave, all LC coordinates in registers.
-template <int LC>
-__device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, const GdItem it,
-                                             const int lane) {
-  const GD_AS_CONST GdJob &J = gd_c(jobs)[it.job];
-  const int Rp = J.Rp, P = J.P, L = J.L, nP = J.nP;
-  const int r = it.rb + lane;  // < Rp by construction (Rp padded to 64)
-  const int l0 = it.a * LC;
-  const int nl = min(LC, L - l0);
-  const int p0 = it.b * J.ptile;
-  const int p1 = min(P, p0 + J.ptile);
-  const GD_AS_CONST float4 *__restrict__ pattr = gd_c(J.pattr);
-  const GD_AS_CONST float *__restrict__ ptab = gd_c(J.ptab);
-  const GD_AS_CONST float *__restrict__ pspv = gd_c(J.pspv);
-  const GD_AS_CONST GdAtom *__restrict__ atoms = gd_c(J.atoms);
-  const GD_AS_GLOBAL float *__restrict__ coords = gd_g(J.coords);
 
-  // ligand chunk: coordinates of this lane's replica, per-atom constants (uniform)
-  float lx[LC], ly[LC], lz[LC], ele[LC], hpc[LC];
-  int tid[LC];
-#pragma unroll
-  for (int j = 0; j < LC; ++j) {
-    const int l = l0 + min(j, nl - 1);
-    lx[j] = coords[(size_t)(3 * l + 0) * Rp + r];
-    ly[j] = coords[(size_t)(3 * l + 1) * Rp + r];
-    lz[j] = coords[(size_t)(3 * l + 2) * Rp + r];
-    tid[j] = atoms[l].tidx * GD_PTAB_PT;
-    ele[j] = 0.0f;
-    hpc[j] = 0.0f;
-  }
-  const GD_AS_GLOBAL float4 *__restrict__ pc = gd_g(J.pcoord) + (size_t)gd_g(J.prti)[r] * P;
-  const float lj0 = J.lj0, lj01 = J.lj0 * J.lj1;
-  // electrostatics carried as el1 * g: el1 (el0 + a1 s^2 + b1 s^3) with s = el1 r
-  // is a polynomial in r, and el1 / s = 1 / r; the sum is scaled by 1/el1 once
-  const float el1 = J.el1;
-  const float e0 = J.el0 * el1, e2 = J.a1 * el1 * el1 * el1, e3 = J.b1 * el1 * el1 * el1 * el1;
-  const float inv_el1 = J.inv_el1;
+// Per-item constants of the pair loop.
+struct GdPairConst {
+  float lj0, lj01;           // LJ denominator lj0 (1 + lj1 r^2) = lj0 + lj0 lj1 r^2
+  float e0, e2, e3;          // el1 * (el0 + a1 s^2 + b1 s^3), s = el1 r, as a polynomial in r
+  float inv_el1;
+};
+
+// The point loop of one item: points b, b + nP, ... against the LC chunk atoms
+// (padding atoms of a partial chunk contribute exactly zero).
+template <int LC>
+__device__ __forceinline__ void gd_pair_points(
+    const GD_AS_CONST GdJob &J, const int b, const GD_AS_GLOBAL float4 *__restrict__ pc,
+    const GdPairConst &k, const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC],
+    const unsigned (&toff)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
+    float &epsp, float &ehdb) {
+  const int P = J.P, nP = J.nP;
+  const GD_AS_GLOBAL float4 *__restrict__ pattr = gd_g(J.pattr);
+  const GD_AS_CONST char *__restrict__ ptab = (const GD_AS_CONST char *)gd_c(J.ptab);
+  const GD_AS_CONST float *__restrict__ pspv = gd_c(J.pspv);
   // -log2(e)/2 held in a VGPR: a VALU op reads at most one SGPR or literal
   float pmf_a;
   asm("v_mov_b32 %0, 0xbf38aa3b" : "=v"(pmf_a));
-  float evdw = 0.0f, epmf = 0.0f, epsp = 0.0f, ehdb = 0.0f;
 
-  float4 cn = pc[p0 < p1 ? p0 : 0];
-  for (int p = p0; p < p1; ++p) {
-    const float4 c = cn;
-    if (p + 1 < p1) cn = pc[p + 1];
-    const float4 pa = pattr[p];
-    const int fl = __float_as_int(pa.z);
-    const int pt = fl & GD_PF_TYPE;
+  // tile b of nP takes points b, b + nP, b + 2 nP, ...: every tile gets the same
+  // mix of points near and far from the pocket, so items cost about the same
+  // The point attributes (uniform) go through the vector memory path and are
+  // prefetched one point ahead with the coordinates: as scalar loads of strided
+  // records they missed the scalar cache and stalled the wave.  The empty asm
+  // hides the index's uniformity, which would otherwise turn the load scalar.
+  int pv = b < P ? b : 0;
+  asm("" : "+v"(pv));
+  float4 cn = pc[b < P ? b : 0];
+  float4 an = pattr[pv];
+  for (int p = b; p < P; p += nP) {
+    const float4 c = cn, pa = an;
+    pv += nP;
+    if (p + nP < P) {
+      cn = pc[p + nP];
+      an = pattr[pv];
+    }
+    const int fl = __builtin_amdgcn_readfirstlane(__float_as_int(pa.z));
+    // byte offset of the point type's entry inside a ligand type's row block
+    const unsigned poff = (unsigned)(fl & GD_PF_TYPE) * (GD_PTAB_W * 4);
     const float pele = pa.x, phpp = pa.y;
 #pragma unroll
     for (int j = 0; j < LC; ++j) {
-      if (j < nl) {
-        const GD_AS_CONST float *__restrict__ prm = ptab + (size_t)(tid[j] + pt) * GD_PTAB_W;
+      {
+        const GD_AS_CONST float *__restrict__ prm =
+            (const GD_AS_CONST float *)(ptab + (toff[j] + poff));
         const float p1a = prm[0], np2a = prm[1], pB = prm[2], pC = prm[3];
         const float pmf1 = prm[4], pthr = prm[5], hdb1c = prm[6], nhdb0c = prm[7];
         // the sign bit of the psp threshold flags an active H-bond pair; the asm
@@ -161,7 +158,7 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
           const float r3 = r2 * rr;
           const float r9 = (r3 * r3) * r3;
           const float num = __builtin_fmaf(np2a, r3, p1a);
-          const float den = __builtin_fmaf(p1a, __builtin_fmaf(lj01, r2, lj0), r9);
+          const float den = __builtin_fmaf(p1a, __builtin_fmaf(k.lj01, r2, k.lj0), r9);
           evdw = __builtin_fmaf(num, gd_rcp(den), evdw);
         }
         // contact PMF (:88-91): exponent (6 - r/2)(r - pmf0) log2(e)
@@ -170,14 +167,14 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
         // Far field, decided for the whole wave: below arg = -25, exp2(arg) <
         // 2^-25 and 1 + exp2(arg) rounds to 1, so the PMF term is exactly pmf1;
         // at s = el1 r >= 1 the electrostatic factor is 1/s (scaled: 1/r).
-        const bool ele_near = rr < inv_el1;
+        const bool ele_near = rr < k.inv_el1;
         const bool near = (arg >= GD_PMF_FAR) | ele_near;
         float g = ri;
         if (__ballot(near) != 0) {
           asm volatile("");
           // distance-dependent electrostatics (:72-78), charge applied per atom:
           // el1 g = poly(r) for s = el1 r < 1, else 1/r
-          const float poly = __builtin_fmaf(__builtin_fmaf(e3, rr, e2), r2, e0);
+          const float poly = __builtin_fmaf(__builtin_fmaf(k.e3, rr, k.e2), r2, k.e0);
           g = ele_near ? poly : ri;
           epmf = __builtin_fmaf(pmf1, gd_rcp(1.0f + gd_exp2(arg)), epmf);
         } else {
@@ -191,7 +188,8 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
           asm volatile("");
           if (r2 <= fabsf(pthr)) {
             asm volatile("");
-            epsp += pspv[(size_t)(tid[j] / GD_PTAB_PT) * P + p];
+            const unsigned row = toff[j] / (GD_PTAB_PT * GD_PTAB_W * 4);
+            epsp += pspv[(size_t)row * P + p];
           }
         }
         // hydrogen bond (:116-121) as c exp2(-(r c - hdb0 c)^2), c = hdb1 sqrt(log2(e)/2),
@@ -203,20 +201,73 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
           ehdb = __builtin_fmaf(hdb1c, gd_exp2(-(h * h)), ehdb);
         }
       }
+      // one atom at a time: interleaving atoms raises register use past the
+      // scalar register file (spills) for no gain in issue rate
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
+}
+
+// One work group (4 waves) = one item: replica block it.rb x ligand chunk it.a x
+// the 4 protein tiles b = 4 it.b + wave.  Each wave evaluates its tile; the
+// group sums the 4 tiles' partials in LDS in a fixed order and writes one slot.
+template <int LC>
+__device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, const GdItem it,
+                                             const int lane, const int wave,
+                                             float (*__restrict__ red)[GD_NLIN + LC][64]) {
+  const GD_AS_CONST GdJob &J = gd_c(jobs)[it.job];
+  const int Rp = J.Rp, P = J.P, L = J.L, nP = J.nP;
+  const int b = it.b * GD_GROUP + wave;  // this wave's protein tile
+  const int r = it.rb + lane;  // < Rp by construction (Rp padded to 64)
+  const int l0 = it.a * LC;
+  const int nl = min(LC, L - l0);
+  const GD_AS_CONST GdAtom *__restrict__ atoms = gd_c(J.atoms);
+  const GD_AS_GLOBAL float *__restrict__ coords = gd_g(J.coords);
+
+  // ligand chunk: coordinates of this lane's replica, per-atom constants (uniform)
+  float lx[LC], ly[LC], lz[LC], ele[LC], hpc[LC];
+  unsigned toff[LC];
+#pragma unroll
+  for (int j = 0; j < LC; ++j) {
+    const int l = l0 + min(j, nl - 1);
+    lx[j] = coords[(size_t)(3 * l + 0) * Rp + r];
+    ly[j] = coords[(size_t)(3 * l + 1) * Rp + r];
+    lz[j] = coords[(size_t)(3 * l + 2) * Rp + r];
+    toff[j] = (unsigned)atoms[l0 + j].tidx * (GD_PTAB_PT * GD_PTAB_W * 4);
+    ele[j] = 0.0f;
+    hpc[j] = 0.0f;
+  }
+  const GD_AS_GLOBAL float4 *__restrict__ pc = gd_g(J.pcoord) + (size_t)gd_g(J.prti)[r] * P;
+  GdPairConst k;
+  k.lj0 = J.lj0;
+  k.lj01 = J.lj0 * J.lj1;
+  const float el1 = J.el1;
+  k.e0 = J.el0 * el1;
+  k.e2 = J.a1 * el1 * el1 * el1;
+  k.e3 = J.b1 * el1 * el1 * el1 * el1;
+  k.inv_el1 = J.inv_el1;
+  const float inv_el1 = k.inv_el1;
+  float evdw = 0.0f, epmf = 0.0f, epsp = 0.0f, ehdb = 0.0f;
+  gd_pair_points<LC>(J, b, pc, k, lx, ly, lz, toff, ele, hpc, evdw, epmf, epsp, ehdb);
   ehdb *= GD_HDB_RC;
 
-  // KDE: this tile's share of every chunk atom's anchor bin
+  // KDE: this tile's share of every chunk atom's anchor bin, a contiguous range
+  // [f(b), f(b+1)) with f(b) = floor(n b / nP) evaluated in FP32: f is
+  // monotonic, f(0) = 0 and the last tile ends at n, so the tiles partition the
+  // bin whatever the rounding (contiguous ranges keep the scalar loads in cache)
   float ekde = 0.0f;
   const float kde2l = J.kde2l;
   const GD_AS_CONST float4 *__restrict__ kde = gd_c(J.kde);
+  const float inv_np = 1.0f / (float)nP;
 #pragma unroll
   for (int j = 0; j < LC; ++j) {
-    if (j < nl) {
-      const GdAtom at = atoms[l0 + j];
+    {
+      const GdAtom at = atoms[l0 + j];  // padding atoms: no bins
+      const int nb = at.kde_e - at.kde_b;
+      const int k0 = at.kde_b + gd_uniform((int)((float)(nb * b) * inv_np));
+      const int k1 = b + 1 == nP ? at.kde_e : at.kde_b + gd_uniform((int)((float)(nb * (b + 1)) * inv_np));
       float s = 0.0f;
-      for (int k = at.kde_b + it.b; k < at.kde_e; k += nP) {  // tile b: bins b, b + nP, ...
+      for (int k = k0; k < k1; ++k) {
         const float4 kp = kde[k];
         const float dx = lx[j] - kp.x, dy = ly[j] - kp.y, dz = lz[j] - kp.z;
         const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
@@ -224,6 +275,7 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
       }
       ekde = __builtin_fmaf(s, at.kde_inv_n, ekde);
     }
+    __builtin_amdgcn_sched_barrier(0);
   }
 
   // MCS: this tile's share of the rows, restricted to the chunk atoms
@@ -231,16 +283,14 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
     const int M = J.M, nL = J.nL;
     const GD_AS_CONST float4 *__restrict__ mcs = gd_c(J.mcs);
     GD_AS_GLOBAL float *__restrict__ pm = gd_g(J.part_mcs);
-    for (int m = it.b; m < M; m += nP) {  // tile b: rows b, b + nP, ...
+    for (int m = b; m < M; m += nP) {  // tile b: rows b, b + nP, ...
       float d = 0.0f;
 #pragma unroll
       for (int j = 0; j < LC; ++j) {
-        if (j < nl) {
-          const float4 t = mcs[(size_t)m * L + l0 + j];
-          if (t.w != 0.0f) {
-            const float dx = lx[j] - t.x, dy = ly[j] - t.y, dz = lz[j] - t.z;
-            d += __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-          }
+        const float4 t = mcs[(size_t)m * J.Ls + l0 + j];  // padding atoms: w == 0
+        if (t.w != 0.0f) {
+          const float dx = lx[j] - t.x, dy = ly[j] - t.y, dz = lz[j] - t.z;
+          d += __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         }
       }
       pm[((size_t)m * nL + it.a) * Rp + r] = d;
@@ -250,51 +300,64 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   // partial sums
   float eele = 0.0f;
 #pragma unroll
-  for (int j = 0; j < LC; ++j)
-    if (j < nl) eele = __builtin_fmaf(atoms[l0 + j].q, ele[j], eele);
+  for (int j = 0; j < LC; ++j) eele = __builtin_fmaf(atoms[l0 + j].q, ele[j], eele);  // padding: q = 0
   eele *= inv_el1;
-  const int nS = J.nL * nP;
-  const int slot = it.a * nP + it.b;
-  GD_AS_GLOBAL float *__restrict__ pl = gd_g(J.part_lin) + (size_t)slot * Rp + r;
-  const size_t kst = (size_t)nS * Rp;
-  pl[0 * kst] = evdw;
-  pl[1 * kst] = eele;
-  pl[2 * kst] = epmf;
-  pl[3 * kst] = epsp;
-  pl[4 * kst] = ehdb;
-  pl[5 * kst] = ekde;
-  GD_AS_GLOBAL float *__restrict__ ph = gd_g(J.part_hpc);
+  // group reduction: tiles 4 it.b .. 4 it.b + 3 summed in wave order
+  red[wave][0][lane] = evdw;
+  red[wave][1][lane] = eele;
+  red[wave][2][lane] = epmf;
+  red[wave][3][lane] = epsp;
+  red[wave][4][lane] = ehdb;
+  red[wave][5][lane] = ekde;
 #pragma unroll
-  for (int j = 0; j < LC; ++j)
-    if (j < nl) ph[((size_t)(l0 + j) * nP + it.b) * Rp + r] = hpc[j];
+  for (int j = 0; j < LC; ++j) red[wave][GD_NLIN + j][lane] = hpc[j];
+  __syncthreads();
+  const int nG = J.nG;
+  const size_t nS = (size_t)J.nL * nG;
+  const int slot = it.a * nG + it.b;
+  GD_AS_GLOBAL float *__restrict__ pl = gd_g(J.part_lin);
+  GD_AS_GLOBAL float *__restrict__ ph = gd_g(J.part_hpc);
+  for (int q = wave; q < GD_NLIN + LC; q += GD_GROUP) {
+    const float v = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+    if (q < GD_NLIN)
+      pl[((size_t)q * nS + slot) * Rp + r] = v;
+    else if (q - GD_NLIN < nl)
+      ph[((size_t)(l0 + q - GD_NLIN) * nG + it.b) * Rp + r] = v;
+  }
+  __syncthreads();  // red[] is reused by the next item of a persistent group
 }
 
-// One wave per work item (grid = items / 4 workgroups of 4 waves).
+// One work group of 4 waves per item (grid = items).
 template <int LC>
 __global__ __launch_bounds__(256) void gd_pair_kernel(const GdJob *__restrict__ jobs,
                                                        const GdItem *__restrict__ items,
                                                        int n_items) {
-  const int iid = gd_uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-  if (iid >= n_items) return;
-  gd_pair_item<LC>(jobs, gd_c(items)[iid], threadIdx.x & 63);
+  __shared__ float red[GD_GROUP][GD_NLIN + LC][64];
+  const int iid = gd_uniform((int)blockIdx.x);
+  if (iid >= n_items) return;  // whole group: no wave waits at the barriers
+  gd_pair_item<LC>(jobs, gd_c(items)[iid], threadIdx.x & 63, gd_uniform((int)(threadIdx.x >> 6)), red);
 }
 
-// Persistent form: a fixed grid sized to the machine; every wave pulls item
+// Persistent form: a fixed grid sized to the machine; every group pulls item
 // indices from a device counter until the list is exhausted, which balances
 // items of uneven cost (complexes of different sizes in one launch).  The
 // counter is zeroed by the kernel that runs before each launch (finish or
-// transform kernel); every wave exits once the counter passes n_items.
+// transform kernel); every group exits once the counter passes n_items.
 template <int LC>
 __global__ __launch_bounds__(256) void gd_pair_kernel_wq(const GdJob *__restrict__ jobs,
                                                           const GdItem *__restrict__ items,
                                                           int n_items, int *__restrict__ wq) {
-  const int lane = threadIdx.x & 63;
+  __shared__ float red[GD_GROUP][GD_NLIN + LC][64];
+  __shared__ int s_iid;
+  const int lane = threadIdx.x & 63, wave = gd_uniform((int)(threadIdx.x >> 6));
   for (;;) {
-    int iid = 0;
-    if (lane == 0) iid = atomicAdd(wq, 1);
-    iid = gd_uniform(__shfl(iid, 0));
-    if (iid >= n_items) break;
-    gd_pair_item<LC>(jobs, gd_c(items)[iid], lane);
+    if (threadIdx.x == 0) s_iid = atomicAdd(wq, 1);
+    __syncthreads();
+    const int iid = gd_uniform(s_iid);
+    if (iid >= n_items) break;  // the same value in every wave: the group leaves together
+    // s_iid is rewritten only after the item's barriers, which every wave
+    // reaches after reading it
+    gd_pair_item<LC>(jobs, gd_c(items)[iid], lane, wave, red);
   }
 }
 
@@ -416,7 +479,7 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = bk.rb + lane;
   const int Rp = J.Rp;
-  const int nS = J.nL * J.nP;
+  const int nS = J.nL * J.nG;
   const int step = launch_step - J.step0;  // this job's MC step
 
   // (1) slot reductions: wave w takes the contiguous w-th share of the slots
@@ -431,7 +494,7 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
       acc[k] = gd_sum_strided(J.part_lin + ((size_t)k * nS + q0) * Rp + r, q1 - q0, (size_t)Rp);
   }
   for (int l = wave; l < J.L; l += GD_FW) {  // hpc per atom (energy_prt_v1.C:132-133)
-    const float h = gd_sum_strided(J.part_hpc + (size_t)l * J.nP * Rp + r, J.nP, (size_t)Rp);
+    const float h = gd_sum_strided(J.part_hpc + (size_t)l * J.nG * Rp + r, J.nG, (size_t)Rp);
     const GdAtom at = J.atoms[l];
     const float t = (h - at.hpl0) / at.hpl1;
     acc[6] += 0.5f * t * t - at.hpl2;

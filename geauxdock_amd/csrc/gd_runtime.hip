@@ -141,13 +141,17 @@ bool tuned_tiling(int L, int P, int R, int &lc, int &ptile);  // gd_batch.hip
 
 void default_tiling(int L, int P, int R, int &lc, int &ptile) {
   if (tuned_tiling(L, P, R, lc, ptile)) return;
-  lc = L <= 12 ? 4 : 8;
+  // 4 atoms per wave keeps the pair kernel at ~55 VGPRs (8+ waves per SIMD);
+  // small point tiles give many equal-cost waves, and the group reduction keeps
+  // the partial-sum volume independent of how many tiles a group covers
+  lc = 4;
   const int nL = (L + lc - 1) / lc;
   const int nblk = (R + 63) / 64;
   const int simds = 4 * (g_cu > 0 ? g_cu : 256);
-  const int target_waves = 4 * simds;
+  const int target_waves = 32 * simds;
   int nP = std::max(1, target_waves / std::max(1, nblk * nL));
   nP = std::min(nP, std::max(1, P / 16));
+  nP = (nP + GD_GROUP - 1) / GD_GROUP * GD_GROUP;
   ptile = (P + nP - 1) / nP;
 }
 
@@ -158,9 +162,9 @@ static gd_status alloc_partials(gd_job *j, GdJob &g, int cap_rp) {
     j->d_part = nullptr;
   }
   Arena a;
-  const size_t nS = (size_t)g.nL * g.nP;
+  const size_t nS = (size_t)g.nL * g.nG;
   a.add(&g.part_lin, (size_t)GD_NLIN * nS * cap_rp);
-  a.add(&g.part_hpc, (size_t)g.L * g.nP * cap_rp);
+  a.add(&g.part_hpc, (size_t)g.L * g.nG * cap_rp);
   a.add(&g.part_mcs, (size_t)std::max(1, g.M) * g.nL * cap_rp);
   return a.commit(&j->d_part);
 }
@@ -172,13 +176,17 @@ gd_status apply_tiling(gd_job *j, int lc, int ptile) {
   g.lc = lc;
   g.nL = (g.L + lc - 1) / lc;
   g.ptile = ptile;
-  g.nP = (g.P + ptile - 1) / ptile;
+  // protein tiles: points p = b, b + nP, ...; a multiple of GD_GROUP so that a
+  // work group's waves take tiles 4g .. 4g + 3 of the same chunk
+  const int nP = (g.P + ptile - 1) / ptile;
+  g.nP = (nP + GD_GROUP - 1) / GD_GROUP * GD_GROUP;
+  g.nG = g.nP / GD_GROUP;
   gd_status st = alloc_partials(j, g, g.Rp);
   if (st != GD_OK) return st;
-  j->items.clear();
+  j->items.clear();  // one item per work group: (replica block, chunk, tile group)
   for (int rb = 0; rb < g.Rp; rb += 64)
-    for (int b = 0; b < g.nP; ++b)
-      for (int a = 0; a < g.nL; ++a) j->items.push_back({0, rb, a, b});
+    for (int gi = 0; gi < g.nG; ++gi)
+      for (int a = 0; a < g.nL; ++a) j->items.push_back({0, rb, a, gi});
   return GD_OK;
 }
 
@@ -221,7 +229,14 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
     }
   const int nty = (int)types.size();
   g.n_types = nty;
-  std::vector<float> ptab((size_t)nty * GD_PTAB_PT * GD_PTAB_W, 0.0f);
+  // atom stride padded to 16 (the widest ligand chunk): the pair kernel runs whole
+  // chunks; padding atoms use row block nty, all zero with a NaN psp threshold
+  // (no psp, no H-bond), charge 0, no KDE bins and no MCS entries
+  const int Ls = (L + 15) / 16 * 16;
+  g.Ls = Ls;
+  std::vector<float> ptab((size_t)(nty + 1) * GD_PTAB_PT * GD_PTAB_W, 0.0f);
+  for (int pt = 0; pt < GD_PTAB_PT; ++pt)
+    ptab[((size_t)nty * GD_PTAB_PT + pt) * GD_PTAB_W + 5] = std::numeric_limits<float>::quiet_NaN();
   const double log2e = 1.44269504088896340736;
   const double hdb_c = std::sqrt(0.5 * log2e);
   for (int ti = 0; ti < nty; ++ti) {
@@ -247,7 +262,7 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       o[7] = hb ? (float)(-(double)e->hdb0[lt][pt] * c) : 0.0f;
     }
   }
-  std::vector<float> pspv((size_t)nty * P, 0.0f);
+  std::vector<float> pspv((size_t)(nty + 1) * P, 0.0f);  // last row: padding atoms (zero)
   std::vector<float4> pattr(P);
   for (int p = 0; p < P; ++p) {
     const int c = d->prt_c[p];
@@ -271,7 +286,13 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
     }
     kde[k] = make_float4(d->kde_x[k], d->kde_y[k], d->kde_z[k], 0.0f);
   }
-  std::vector<GdAtom> atoms(L);
+  std::vector<GdAtom> atoms(Ls);
+  for (int l = L; l < Ls; ++l) {
+    GdAtom &a = atoms[l];
+    std::memset(&a, 0, sizeof a);
+    a.tidx = nty;
+    a.hpl1 = 1.0f;
+  }
   for (int l = 0; l < L; ++l) {
     const int lt = d->lig_t[l];
     GdAtom &a = atoms[l];
@@ -288,7 +309,7 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
     a.hpl1 = e->hpl1[lt];
     a.hpl2 = e->hpl2[lt];
   }
-  std::vector<float4> mcs((size_t)std::max(1, M) * L);
+  std::vector<float4> mcs((size_t)std::max(1, M) * Ls, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
   std::vector<float2> mcsrow(std::max(1, M));
   for (int m = 0; m < M; ++m) {
     int n = 0;
@@ -296,8 +317,7 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       const size_t i = (size_t)m * L + l;
       const bool v = d->mcs_x[i] != GD_MCS_INVALID;
       n += v;
-      mcs[i] = v ? make_float4(d->mcs_x[i], d->mcs_y[i], d->mcs_z[i], 1.0f)
-                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (v) mcs[(size_t)m * Ls + l] = make_float4(d->mcs_x[i], d->mcs_y[i], d->mcs_z[i], 1.0f);
     }
     mcsrow[m] = make_float2(d->mcs_tcc[m], (float)n);
   }
@@ -391,16 +411,19 @@ size_t footprint(const gd_job_desc *d) {
   size_t nty = 0;
   for (size_t l = 0; l < L; ++l)
     if (d->lig_t[l] >= 0 && d->lig_t[l] < GD_NLT && !seen[d->lig_t[l]]++) ++nty;
-  size_t b = a(nl * L * 16) + a(L * sizeof(GdAtom)) + a(np * P * 16) + a(P * 16) +
-             a(nty * GD_PTAB_PT * GD_PTAB_W * 4) + a(nty * P * 4) + a(std::max<size_t>(1, K) * 16) +
-             a(std::max<size_t>(1, M) * L * 16) + a(std::max<size_t>(1, M) * 8) + a(nt * 4);
+  const size_t Ls = (L + 15) / 16 * 16;
+  size_t b = a(nl * L * 16) + a(Ls * sizeof(GdAtom)) + a(np * P * 16) + a(P * 16) +
+             a((nty + 1) * GD_PTAB_PT * GD_PTAB_W * 4) + a((nty + 1) * P * 4) +
+             a(std::max<size_t>(1, K) * 16) + a(std::max<size_t>(1, M) * Ls * 16) +
+             a(std::max<size_t>(1, M) * 8) + a(nt * 4);
   b += a(6 * Rp * 4) * 2 + a(10 * Rp * 4) + a(Rp * 4) * 6 + a(Rp * 4) + a(nl * np * nt * 4) +
        a(3 * L * Rp * 4) + a(Rp * 4) + a(10 * Rp * 4) + a(9 * Rp * 4) + a(64 * 4);
   int lc, ptile;
   default_tiling((int)L, (int)P, (int)R, lc, ptile);
-  const size_t nL = (L + lc - 1) / lc, nP = (P + ptile - 1) / ptile;
-  b += a(GD_NLIN * nL * nP * Rp * 4) + a(L * nP * Rp * 4) + a(std::max<size_t>(1, M) * nL * Rp * 4);
-  b += (Rp / 64) * nL * nP * sizeof(GdItem) + (Rp / 64) * sizeof(GdBlock) + 512;
+  const size_t nL = (L + lc - 1) / lc, nP0 = (P + ptile - 1) / ptile;
+  const size_t nG = (nP0 + GD_GROUP - 1) / GD_GROUP;
+  b += a(GD_NLIN * nL * nG * Rp * 4) + a(L * nG * Rp * 4) + a(std::max<size_t>(1, M) * nL * Rp * 4);
+  b += (Rp / 64) * nL * nG * sizeof(GdItem) + (Rp / 64) * sizeof(GdBlock) + 512;
   return b;
 }
 
@@ -464,7 +487,7 @@ gd_status sync_job_struct(gd_job *j) {
 gd_status launch_pair(int lc, const GdJob *d_jobs, const GdItem *d_items, int n_items,
                       hipStream_t s) {
   if (n_items <= 0) return GD_OK;
-  const dim3 grid((n_items + 3) / 4), block(256);
+  const dim3 grid(n_items), block(256);  // one 4-wave group per item
   switch (lc) {
     case 4:
       hipLaunchKernelGGL(gd_pair_kernel<4>, grid, block, 0, s, d_jobs, d_items, n_items);
@@ -671,7 +694,7 @@ size_t gd_job_footprint(const gd_job_desc *d) { return footprint(d); }
 size_t gd_job_device_bytes(const gd_job *j) {
   if (!j) return 0;
   const GdJob &g = j->h;
-  const size_t part = ((size_t)GD_NLIN * g.nL * g.nP + (size_t)g.L * g.nP + (size_t)g.M * g.nL) *
+  const size_t part = ((size_t)GD_NLIN * g.nL * g.nG + (size_t)g.L * g.nG + (size_t)g.M * g.nL) *
                       g.Rp * sizeof(float);
   return j->bytes + part;
 }
