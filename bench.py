@@ -92,8 +92,8 @@ def batched(kind, n_jobs, mc_steps, plain_jobs):
     hosts = [g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.004,
                              ceiling_temp=0.04, seed=100 + i) for i, z in enumerate(sizes)]
     devs = [g.DeviceJob(h) for h in hosts]
-    for d in devs:                       # coarse batch tiling: 8 atoms x <= 1024 points
-        d.set_tiling(8, min(1024, d.host.desc.prt_npoint))
+    for d in devs:                       # shared-launch tiling: 4 atoms x 64 points
+        d.set_tiling(4, min(64, d.host.desc.prt_npoint))
     mcs = [g.make_mc(mc_steps, steps_per_exchange=100, move_mode="gaussian", seed=5, job_id=i)
            for i in range(n_jobs)]
     arr = (C.c_void_p * n_jobs)(*[d.h.value for d in devs])

@@ -35,7 +35,7 @@ namespace {
 hipError_t launch_pair_wq(int lc, const GdJob *jobs, const GdItem *items, int n_items, int *wq,
                           int grid, hipStream_t s) {
   if (n_items <= 0) return hipSuccess;
-  const dim3 g(grid), b(256);
+  const dim3 g(grid), b(64 * GD_GROUP);
   switch (lc) {
     case 4:
       hipLaunchKernelGGL(gd_pair_kernel_wq<4>, g, b, 0, s, jobs, items, n_items, wq);
@@ -110,10 +110,21 @@ struct BatchLists {
   }
 };
 
-int pair_grid(int waves_per_cu) {
+int pair_grid(int waves_per_cu) {  // work groups of GD_GROUP waves
   const int cu = std::max(1, n_cu());
-  return std::max(1, cu * std::max(4, waves_per_cu) / 4);
+  return std::max(1, cu * std::max(GD_GROUP, waves_per_cu) / GD_GROUP);
 }
+
+// Resident waves per CU of the persistent pair kernel: as many 4-wave groups as
+// the kernel's registers allow (4 atoms: ~60 VGPRs, 8 waves per SIMD).
+int pair_waves_per_cu(int lc) { return lc == 4 ? 32 : lc == 8 ? 20 : 8; }
+
+// Tiling of a job inside shared launches (batches, serving): a function of the
+// complex only, so a job's bits do not depend on what it runs with.  Tiles of
+// 64 points keep the per-item fixed cost small; the launch holds items of many
+// jobs, so the machine is full without small tiles.
+constexpr int kSharedLc = 4;
+int shared_ptile(int P) { return std::min(P, 64); }
 
 // One MC step of every job in the lists (pair + finish), optional exchange.
 gd_status batch_step(const BatchLists &B, int lc, int grid, int launch_step, const int *base,
@@ -149,11 +160,11 @@ gd_status batch_init(const BatchLists &B, int lc, int grid, hipStream_t s) {
 
 // Bring every job to one ligand-chunk width (one kernel template per launch).
 gd_status common_lc(const std::vector<gd_job *> &jobs, int &lc) {
-  lc = jobs.empty() ? 8 : jobs[0]->h.lc;
+  lc = jobs.empty() ? kSharedLc : jobs[0]->h.lc;
   bool same = true;
   for (gd_job *j : jobs) same = same && j->h.lc == lc;
   if (same) return GD_OK;
-  lc = 8;
+  lc = kSharedLc;
   for (gd_job *j : jobs)
     if (j->h.lc != lc) {
       gd_status st = apply_tiling(j, lc, j->h.ptile);
@@ -192,7 +203,7 @@ extern "C" gd_status gd_batch_dock(gd_job **jobs_in, int n_jobs, const gd_mc_par
   for (int k = 0; k < n_jobs; ++k) active[k] = k;
   BatchLists B;
   if ((st = B.build(jobs, active, s)) != GD_OK) return st;
-  const int grid = pair_grid(16);
+  const int grid = pair_grid(pair_waves_per_cu(lc));
   KTimer tp;
   const auto t0 = std::chrono::steady_clock::now();
   if ((st = batch_init(B, lc, grid, s)) != GD_OK) return st;
@@ -356,13 +367,14 @@ extern "C" gd_status gd_autotune(gd_job *j, int steps, gd_tiling *best, double *
   const GdJob g0 = j->h;
   const int L = g0.L, P = g0.P;
   std::vector<std::pair<int, int>> cand;
-  for (int lc : {4, 8, 16}) {
-    if (lc == 16 && L < 16) continue;
-    for (int pt : {128, 256, 512, 1024, 2048})
-      if (pt <= std::max(128, P)) cand.push_back({lc, std::min(pt, P)});
+  for (int lc : {4, 8}) {
+    for (int pt : {13, 20, 26, 39, 52, 78, 104, 156})
+      if (pt <= P) cand.push_back({lc, pt});
+  }
+  {
     int dlc, dpt;
     default_tiling(L, P, g0.R, dlc, dpt);
-    cand.push_back({lc, dpt});
+    cand.push_back({dlc, dpt});
   }
   gd_mc_param mc;
   std::memset(&mc, 0, sizeof mc);
@@ -393,7 +405,7 @@ extern "C" gd_status gd_autotune(gd_job *j, int steps, gd_tiling *best, double *
   if (best) {
     best->atoms_per_lane = bestc.first;
     best->tile_points = bestc.second;
-    best->waves_per_cu = 16;
+    best->waves_per_cu = pair_waves_per_cu(bestc.first);
   }
   if (best_ms) *best_ms = best_t;
   return GD_OK;
@@ -428,7 +440,7 @@ struct gd_server {
   hipStream_t stream = nullptr;
   BatchLists lists;
   bool dirty = true;
-  int lc = 8;
+  int lc = kSharedLc;
   std::chrono::steady_clock::time_point t0;
   std::mutex mu;
   double now() const {
@@ -552,10 +564,9 @@ extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
       delete j;
       continue;
     }
-    // serving tiling: a function of the complex only (coarse protein tiles;
-    // the shared launch has items of many jobs to fill the machine)
+    // serving tiling: a function of the complex only
     {
-      if ((st = apply_tiling(j->job, s->lc, std::min(1024, j->job->h.P))) != GD_OK ||
+      if ((st = apply_tiling(j->job, s->lc, shared_ptile(j->job->h.P))) != GD_OK ||
           (st = sync_job_struct(j->job)) != GD_OK) {
         s->results.push_back({j->ticket, st, j->t_submit, s->now(), s->now(), 0.0});
         gd_job_destroy(j->job);
@@ -584,7 +595,7 @@ extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
     }
     BatchLists B;
     if ((st = B.build(nj, idx, hs)) != GD_OK) return st;
-    if ((st = batch_init(B, s->lc, pair_grid(16), hs)) != GD_OK) return st;
+    if ((st = batch_init(B, s->lc, pair_grid(pair_waves_per_cu(s->lc)), hs)) != GD_OK) return st;
     hipError_t e = hipStreamSynchronize(hs);
     if (e != hipSuccess) return hip_fail(e, "init pass");
   }
@@ -608,7 +619,7 @@ extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
       if (spe > 0 && j->job->h.n_tmp > 1 && (j->done_steps + 1) % spe == 0)
         ex.push_back({q, (j->done_steps + 1) / spe - 1});
     }
-    st = batch_step(s->lists, s->lc, pair_grid(16), s->launch_step, nullptr, &ex, hs, nullptr);
+    st = batch_step(s->lists, s->lc, pair_grid(pair_waves_per_cu(s->lc)), s->launch_step, nullptr, &ex, hs, nullptr);
     if (!ex.empty()) (void)hipStreamSynchronize(hs);  // ex table is reused next step
     ++s->launch_step;
     for (int q : act) s->active[q]->done_steps += 1;

@@ -20,7 +20,7 @@
 #define GD_PTAB_PT 32     // point-type stride of the table (31 used)
 #define GD_REC_W 17       // floats per accept record: step, movematrix[6], energy[10]
 #define GD_NLIN 6
-#define GD_GROUP 4        // protein tiles per work group (summed in LDS before writing)         // linear partial sums: vdw ele pmf psp hdb kde
+#define GD_GROUP 8        // protein tiles (waves) per pair-kernel work group, summed in LDS         // linear partial sums: vdw ele pmf psp hdb kde
 
 // point flag bits (pattr.z as int)
 #define GD_PF_TYPE 0x1f
@@ -48,6 +48,7 @@ struct GdJob {
   int n_lig, n_prt, n_tmp, R, Rp;
   int nL, nP, ptile;      // ligand chunks, protein tiles (a multiple of GD_GROUP), points per tile
   int nG;                 // tile groups = nP / GD_GROUP: the partial-sum slots per chunk
+  int pcls[5];            // points [pcls[c], pcls[c+1]) have class c = cdc + 2 * (class == 2)
   int lc;                 // ligand atoms per chunk (kernel template value)
   int n_types;            // rows of the parameter table
   // constant tables (device)
@@ -75,7 +76,6 @@ struct GdJob {
   float *coords;          // [L][3][Rp] transformed ligand of the current proposal
   float *part_lin;        // [GD_NLIN][nL*nG][Rp]
   float *part_hpc;        // [L][nG][Rp]
-  float *part_mcs;        // [M][nL][Rp]
   float *rec;             // [Rp][rec_cap][GD_REC_W] accept records (optional)
   int *rec_n;             // [Rp]
   int rec_cap;

@@ -84,133 +84,157 @@ struct GdPairConst {
   float inv_el1;
 };
 
-// The point loop of one item: points b, b + nP, ... against the LC chunk atoms
-// (padding atoms of a partial chunk contribute exactly zero).
-template <int LC>
-__device__ __forceinline__ void gd_pair_points(
-    const GD_AS_CONST GdJob &J, const int b, const GD_AS_GLOBAL float4 *__restrict__ pc,
-    const GdPairConst &k, const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC],
-    const unsigned (&toff)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
-    float &epsp, float &ehdb) {
+// The LC chunk atoms against one point.  CLS: bit 0 = cdc point (hydrophobic
+// term), bit 1 = class-2 point (psp term); the caller dispatches on the point's
+// flags once, so the atoms carry no per-pair class tests.
+template <int LC, int CLS>
+__device__ __forceinline__ void gd_pair_atoms(
+    const GdPairConst &k, const GD_AS_CONST char *__restrict__ ptab,
+    const GD_AS_CONST float *__restrict__ pspv, const int P, const int p, const unsigned poff,
+    const float4 c, const float pele, const float phpp, const float pmf_a, const float (&lx)[LC],
+    const float (&ly)[LC], const float (&lz)[LC], const unsigned (&toff)[LC], float (&ele)[LC],
+    float (&hpc)[LC], float &evdw, float &epmf, float &epsp, float &ehdb) {
+#pragma unroll
+  for (int j = 0; j < LC; ++j) {
+    const GD_AS_CONST float *__restrict__ prm =
+        (const GD_AS_CONST float *)(ptab + (toff[j] + poff));
+    const float p1a = prm[0], np2a = prm[1], pB = prm[2], pC = prm[3];
+    const float pmf1 = prm[4], pthr = prm[5], hdb1c = prm[6], nhdb0c = prm[7];
+    // the sign bit of the psp threshold flags an active H-bond pair; the asm
+    // keeps the test on the scalar unit (as a float compare it became a VALU op)
+    int hflag;
+    asm("" : "=s"(hflag) : "0"(__float_as_int(pthr)));
+    const float dx = lx[j] - c.x, dy = ly[j] - c.y, dz = lz[j] - c.z;
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    const float ri = gd_rsq(r2);
+    const float rr = r2 * ri;
+    // hydrophobic contact (energy_prt_v1.C:44-50):
+    // hpp (1 - r^2 (a - b r^2 + c r^4 - d r^6)) for r^2 <= 81, on the lanes
+    // within 9 A only (exec-masked; skipped when no lane is).  The empty
+    // volatile asm keeps the branch: without it the compiler evaluates the
+    // term for every pair and selects
+    if (CLS & 1) {
+      if (r2 <= 81.0f) {
+        asm volatile("");
+        float w = __builtin_fmaf(-GD_HPD, r2, GD_HPC);
+        w = __builtin_fmaf(w, r2, -GD_HPB);
+        w = __builtin_fmaf(w, r2, GD_HPA);
+        hpc[j] = __builtin_fmaf(phpp, __builtin_fmaf(-r2, w, 1.0f), hpc[j]);
+      }
+    }
+    // soft Lennard-Jones (:61-64): (p1 - p2) / (p1 lj0 (1 + lj1 r^2) + 1) with
+    // p1 = p1a / r^9, p2 = p2a / r^6, multiplied through by r^9:
+    // (p1a - p2a r^3) / (p1a lj0 (1 + lj1 r^2) + r^9)
+    {
+      const float r3 = r2 * rr;
+      const float r9 = (r3 * r3) * r3;
+      const float num = __builtin_fmaf(np2a, r3, p1a);
+      const float den = __builtin_fmaf(p1a, __builtin_fmaf(k.lj01, r2, k.lj0), r9);
+      evdw = __builtin_fmaf(num, gd_rcp(den), evdw);
+    }
+    // contact PMF (:88-91): exponent (6 - r/2)(r - pmf0) log2(e)
+    // = -log2(e)/2 r^2 + pB r + pC
+    const float arg = __builtin_fmaf(pB, rr, __builtin_fmaf(pmf_a, r2, pC));
+    // Far field, decided for the whole wave: below arg = -25, exp2(arg) <
+    // 2^-25 and 1 + exp2(arg) rounds to 1, so the PMF term is exactly pmf1;
+    // at s = el1 r >= 1 the electrostatic factor is 1/s (scaled: 1/r).
+    const bool ele_near = rr < k.inv_el1;
+    const bool near = (arg >= GD_PMF_FAR) | ele_near;
+    float g = ri;
+    if (__ballot(near) != 0) {
+      asm volatile("");
+      // distance-dependent electrostatics (:72-78), charge applied per atom:
+      // el1 g = poly(r) for s = el1 r < 1, else 1/r
+      const float poly = __builtin_fmaf(__builtin_fmaf(k.e3, rr, k.e2), r2, k.e0);
+      g = ele_near ? poly : ri;
+      epmf = __builtin_fmaf(pmf1, gd_rcp(1.0f + gd_exp2(arg)), epmf);
+    } else {
+      epmf += pmf1;
+    }
+    ele[j] = __builtin_fmaf(pele, g, ele[j]);
+    // pocket-specific potential (:100-107): r <= pmf0 decided on r^2
+    // against the exact float threshold of sqrtf(r^2) <= pmf0; the value is
+    // loaded only when some lane is in contact (exec-masked branch)
+    if (CLS & 2) {
+      if (r2 <= fabsf(pthr)) {
+        asm volatile("");
+        const unsigned row = toff[j] / (GD_PTAB_PT * GD_PTAB_W * 4);
+        epsp += pspv[(size_t)row * P + p];
+      }
+    }
+    // hydrogen bond (:116-121) as c exp2(-(r c - hdb0 c)^2), c = hdb1 sqrt(log2(e)/2),
+    // summed without the 1/sqrt(log2(e)/2) factor; pairs with hdb0 <= 0.1 skip
+    // the term on a wave-uniform branch
+    if (hflag < 0) {
+      asm volatile("");
+      const float h = __builtin_fmaf(rr, hdb1c, nhdb0c);
+      ehdb = __builtin_fmaf(hdb1c, gd_exp2(-(h * h)), ehdb);
+    }
+    // one atom at a time: interleaving atoms raises register use past the
+    // scalar register file (spills) for no gain in issue rate
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// The points of one class [lo, hi) of one tile against the LC chunk atoms:
+// tile b of nP takes lo + b, lo + b + nP, ...  Every tile gets the same share of
+// every class and the same mix of points near and far from the pocket, so
+// waves cost about the same.
+template <int LC, int CLS>
+__device__ __forceinline__ void gd_pair_class(
+    const GD_AS_CONST GdJob &J, const int lo, const int hi, const int b,
+    const GD_AS_GLOBAL float4 *__restrict__ pc, const GdPairConst &k, const float pmf_a,
+    const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC], const unsigned (&toff)[LC],
+    float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf, float &epsp, float &ehdb) {
   const int P = J.P, nP = J.nP;
   const GD_AS_GLOBAL float4 *__restrict__ pattr = gd_g(J.pattr);
   const GD_AS_CONST char *__restrict__ ptab = (const GD_AS_CONST char *)gd_c(J.ptab);
   const GD_AS_CONST float *__restrict__ pspv = gd_c(J.pspv);
-  // -log2(e)/2 held in a VGPR: a VALU op reads at most one SGPR or literal
-  float pmf_a;
-  asm("v_mov_b32 %0, 0xbf38aa3b" : "=v"(pmf_a));
-
-  // tile b of nP takes points b, b + nP, b + 2 nP, ...: every tile gets the same
-  // mix of points near and far from the pocket, so items cost about the same
+  const int p0 = lo + b;
   // The point attributes (uniform) go through the vector memory path and are
   // prefetched one point ahead with the coordinates: as scalar loads of strided
   // records they missed the scalar cache and stalled the wave.  The empty asm
   // hides the index's uniformity, which would otherwise turn the load scalar.
-  int pv = b < P ? b : 0;
+  int pv = p0 < hi ? p0 : 0;
   asm("" : "+v"(pv));
-  float4 cn = pc[b < P ? b : 0];
+  float4 cn = pc[p0 < hi ? p0 : 0];
   float4 an = pattr[pv];
-  for (int p = b; p < P; p += nP) {
+  for (int p = p0; p < hi; p += nP) {
     const float4 c = cn, pa = an;
     pv += nP;
-    if (p + nP < P) {
+    if (p + nP < hi) {
       cn = pc[p + nP];
       an = pattr[pv];
     }
     const int fl = __builtin_amdgcn_readfirstlane(__float_as_int(pa.z));
     // byte offset of the point type's entry inside a ligand type's row block
     const unsigned poff = (unsigned)(fl & GD_PF_TYPE) * (GD_PTAB_W * 4);
-    const float pele = pa.x, phpp = pa.y;
-#pragma unroll
-    for (int j = 0; j < LC; ++j) {
-      {
-        const GD_AS_CONST float *__restrict__ prm =
-            (const GD_AS_CONST float *)(ptab + (toff[j] + poff));
-        const float p1a = prm[0], np2a = prm[1], pB = prm[2], pC = prm[3];
-        const float pmf1 = prm[4], pthr = prm[5], hdb1c = prm[6], nhdb0c = prm[7];
-        // the sign bit of the psp threshold flags an active H-bond pair; the asm
-        // keeps the test on the scalar unit (as a float compare it became a VALU op)
-        int hflag;
-        asm("" : "=s"(hflag) : "0"(__float_as_int(pthr)));
-        const float dx = lx[j] - c.x, dy = ly[j] - c.y, dz = lz[j] - c.z;
-        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-        const float ri = gd_rsq(r2);
-        const float rr = r2 * ri;
-        // hydrophobic contact (energy_prt_v1.C:44-50):
-        // hpp (1 - r^2 (a - b r^2 + c r^4 - d r^6)) for r^2 <= 81, on the lanes
-        // within 9 A only (exec-masked; skipped when no lane is).  The empty
-        // volatile asm statements keep both branches: without them the compiler
-        // evaluates the compare, or the whole term, for every pair
-        if (fl & GD_PF_CDC) {
-          asm volatile("");
-          if (r2 <= 81.0f) {
-            asm volatile("");
-            float w = __builtin_fmaf(-GD_HPD, r2, GD_HPC);
-            w = __builtin_fmaf(w, r2, -GD_HPB);
-            w = __builtin_fmaf(w, r2, GD_HPA);
-            hpc[j] = __builtin_fmaf(phpp, __builtin_fmaf(-r2, w, 1.0f), hpc[j]);
-          }
-        }
-        // soft Lennard-Jones (:61-64): (p1 - p2) / (p1 lj0 (1 + lj1 r^2) + 1) with
-        // p1 = p1a / r^9, p2 = p2a / r^6, multiplied through by r^9:
-        // (p1a - p2a r^3) / (p1a lj0 (1 + lj1 r^2) + r^9)
-        {
-          const float r3 = r2 * rr;
-          const float r9 = (r3 * r3) * r3;
-          const float num = __builtin_fmaf(np2a, r3, p1a);
-          const float den = __builtin_fmaf(p1a, __builtin_fmaf(k.lj01, r2, k.lj0), r9);
-          evdw = __builtin_fmaf(num, gd_rcp(den), evdw);
-        }
-        // contact PMF (:88-91): exponent (6 - r/2)(r - pmf0) log2(e)
-        // = -log2(e)/2 r^2 + pB r + pC
-        const float arg = __builtin_fmaf(pB, rr, __builtin_fmaf(pmf_a, r2, pC));
-        // Far field, decided for the whole wave: below arg = -25, exp2(arg) <
-        // 2^-25 and 1 + exp2(arg) rounds to 1, so the PMF term is exactly pmf1;
-        // at s = el1 r >= 1 the electrostatic factor is 1/s (scaled: 1/r).
-        const bool ele_near = rr < k.inv_el1;
-        const bool near = (arg >= GD_PMF_FAR) | ele_near;
-        float g = ri;
-        if (__ballot(near) != 0) {
-          asm volatile("");
-          // distance-dependent electrostatics (:72-78), charge applied per atom:
-          // el1 g = poly(r) for s = el1 r < 1, else 1/r
-          const float poly = __builtin_fmaf(__builtin_fmaf(k.e3, rr, k.e2), r2, k.e0);
-          g = ele_near ? poly : ri;
-          epmf = __builtin_fmaf(pmf1, gd_rcp(1.0f + gd_exp2(arg)), epmf);
-        } else {
-          epmf += pmf1;
-        }
-        ele[j] = __builtin_fmaf(pele, g, ele[j]);
-        // pocket-specific potential (:100-107): r <= pmf0 decided on r^2
-        // against the exact float threshold of sqrtf(r^2) <= pmf0; the value is
-        // loaded only when some lane is in contact (exec-masked branch)
-        if (fl & GD_PF_CLS2) {
-          asm volatile("");
-          if (r2 <= fabsf(pthr)) {
-            asm volatile("");
-            const unsigned row = toff[j] / (GD_PTAB_PT * GD_PTAB_W * 4);
-            epsp += pspv[(size_t)row * P + p];
-          }
-        }
-        // hydrogen bond (:116-121) as c exp2(-(r c - hdb0 c)^2), c = hdb1 sqrt(log2(e)/2),
-        // summed without the 1/sqrt(log2(e)/2) factor; pairs with hdb0 <= 0.1 skip
-        // the term on a wave-uniform branch
-        if (hflag < 0) {
-          asm volatile("");
-          const float h = __builtin_fmaf(rr, hdb1c, nhdb0c);
-          ehdb = __builtin_fmaf(hdb1c, gd_exp2(-(h * h)), ehdb);
-        }
-      }
-      // one atom at a time: interleaving atoms raises register use past the
-      // scalar register file (spills) for no gain in issue rate
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    gd_pair_atoms<LC, CLS>(k, ptab, pspv, P, p, poff, c, pa.x, pa.y, pmf_a, lx, ly, lz, toff, ele, hpc,
+                           evdw, epmf, epsp, ehdb);
   }
 }
 
-// One work group (4 waves) = one item: replica block it.rb x ligand chunk it.a x
-// the 4 protein tiles b = 4 it.b + wave.  Each wave evaluates its tile; the
-// group sums the 4 tiles' partials in LDS in a fixed order and writes one slot.
+// The point loop of one item, class by class (padding atoms of a partial chunk
+// contribute exactly zero).
+template <int LC>
+__device__ __forceinline__ void gd_pair_points(
+    const GD_AS_CONST GdJob &J, const int b, const GD_AS_GLOBAL float4 *__restrict__ pc,
+    const GdPairConst &k, const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC],
+    const unsigned (&toff)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
+    float &epsp, float &ehdb) {
+  // -log2(e)/2 held in a VGPR: a VALU op reads at most one SGPR or literal
+  float pmf_a;
+  asm("v_mov_b32 %0, 0xbf38aa3b" : "=v"(pmf_a));
+  gd_pair_class<LC, 0>(J, J.pcls[0], J.pcls[1], b, pc, k, pmf_a, lx, ly, lz, toff, ele, hpc, evdw, epmf,
+                       epsp, ehdb);
+  gd_pair_class<LC, 1>(J, J.pcls[1], J.pcls[2], b, pc, k, pmf_a, lx, ly, lz, toff, ele, hpc, evdw, epmf,
+                       epsp, ehdb);
+  gd_pair_class<LC, 2>(J, J.pcls[2], J.pcls[3], b, pc, k, pmf_a, lx, ly, lz, toff, ele, hpc, evdw, epmf,
+                       epsp, ehdb);
+  gd_pair_class<LC, 3>(J, J.pcls[3], J.pcls[4], b, pc, k, pmf_a, lx, ly, lz, toff, ele, hpc, evdw, epmf,
+                       epsp, ehdb);
+}
+
 template <int LC>
 __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, const GdItem it,
                                              const int lane, const int wave,
@@ -278,31 +302,12 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
     __builtin_amdgcn_sched_barrier(0);
   }
 
-  // MCS: this tile's share of the rows, restricted to the chunk atoms
-  {
-    const int M = J.M, nL = J.nL;
-    const GD_AS_CONST float4 *__restrict__ mcs = gd_c(J.mcs);
-    GD_AS_GLOBAL float *__restrict__ pm = gd_g(J.part_mcs);
-    for (int m = b; m < M; m += nP) {  // tile b: rows b, b + nP, ...
-      float d = 0.0f;
-#pragma unroll
-      for (int j = 0; j < LC; ++j) {
-        const float4 t = mcs[(size_t)m * J.Ls + l0 + j];  // padding atoms: w == 0
-        if (t.w != 0.0f) {
-          const float dx = lx[j] - t.x, dy = ly[j] - t.y, dz = lz[j] - t.z;
-          d += __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-        }
-      }
-      pm[((size_t)m * nL + it.a) * Rp + r] = d;
-    }
-  }
-
   // partial sums
   float eele = 0.0f;
 #pragma unroll
   for (int j = 0; j < LC; ++j) eele = __builtin_fmaf(atoms[l0 + j].q, ele[j], eele);  // padding: q = 0
   eele *= inv_el1;
-  // group reduction: tiles 4 it.b .. 4 it.b + 3 summed in wave order
+  // group reduction: the group's tiles summed in wave order
   red[wave][0][lane] = evdw;
   red[wave][1][lane] = eele;
   red[wave][2][lane] = epmf;
@@ -318,7 +323,9 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   GD_AS_GLOBAL float *__restrict__ pl = gd_g(J.part_lin);
   GD_AS_GLOBAL float *__restrict__ ph = gd_g(J.part_hpc);
   for (int q = wave; q < GD_NLIN + LC; q += GD_GROUP) {
-    const float v = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+    float v = red[0][q][lane];
+#pragma unroll
+    for (int w = 1; w < GD_GROUP; ++w) v += red[w][q][lane];
     if (q < GD_NLIN)
       pl[((size_t)q * nS + slot) * Rp + r] = v;
     else if (q - GD_NLIN < nl)
@@ -327,9 +334,9 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   __syncthreads();  // red[] is reused by the next item of a persistent group
 }
 
-// One work group of 4 waves per item (grid = items).
+// One work group of GD_GROUP waves per item (grid = items).
 template <int LC>
-__global__ __launch_bounds__(256) void gd_pair_kernel(const GdJob *__restrict__ jobs,
+__global__ __launch_bounds__(64 * GD_GROUP) void gd_pair_kernel(const GdJob *__restrict__ jobs,
                                                        const GdItem *__restrict__ items,
                                                        int n_items) {
   __shared__ float red[GD_GROUP][GD_NLIN + LC][64];
@@ -344,7 +351,7 @@ __global__ __launch_bounds__(256) void gd_pair_kernel(const GdJob *__restrict__ 
 // counter is zeroed by the kernel that runs before each launch (finish or
 // transform kernel); every group exits once the counter passes n_items.
 template <int LC>
-__global__ __launch_bounds__(256) void gd_pair_kernel_wq(const GdJob *__restrict__ jobs,
+__global__ __launch_bounds__(64 * GD_GROUP) void gd_pair_kernel_wq(const GdJob *__restrict__ jobs,
                                                           const GdItem *__restrict__ items,
                                                           int n_items, int *__restrict__ wq) {
   __shared__ float red[GD_GROUP][GD_NLIN + LC][64];
@@ -465,6 +472,7 @@ __device__ __forceinline__ float gd_sum_strided(const float *__restrict__ p, int
 // Finish kernel: 1024 threads = 16 waves per 64-replica block; lanes are
 // replicas, waves split the partial slots, atoms and MCS rows.
 #define GD_FW 16
+#define GD_FIN_AT 32  // atoms per LDS stage of the finish kernel's MCS pass
 __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict__ jobs,
                                                           const GdBlock *__restrict__ blocks,
                                                           int mode, int launch_step,
@@ -474,6 +482,7 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
   if (step_base) launch_step += *step_base;  // graph replays: base step in device memory
   __shared__ float red[GD_FW][8][64];
   __shared__ float xf[12][64];
+  __shared__ float stage[3 * GD_FIN_AT * 64];  // transformed atoms of the block, for MCS
   const GdBlock bk = blocks[blockIdx.x];
   const GdJob &J = jobs[bk.job];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -499,10 +508,42 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
     const float t = (h - at.hpl0) / at.hpl1;
     acc[6] += 0.5f * t * t - at.hpl2;
   }
-  for (int m = wave; m < J.M; m += GD_FW) {  // MCS rows (energy_mcs_v1.C:5-29)
-    const float d = gd_sum_strided(J.part_mcs + (size_t)m * J.nL * Rp + r, J.nL, (size_t)Rp);
-    const float2 row = J.mcsrow[m];
-    acc[7] += row.x * sqrtf(d / row.y);
+  // MCS rows (energy_mcs_v1.C:5-29) from the transformed atoms of this step's
+  // proposal, staged in LDS 32 atoms at a time by the whole block; wave w takes
+  // rows 4w .. 4w + 3, 64 + 4w .. 64 + 4w + 3, ...; a row's squared distances
+  // are summed in atom order
+  for (int mb = 0; mb < J.M; mb += 4 * GD_FW) {
+    const int m0 = mb + 4 * wave;
+    const int nm = max(0, min(4, J.M - m0));
+    float d[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int lb = 0; lb < J.L; lb += GD_FIN_AT) {
+      const int na = min(GD_FIN_AT, J.L - lb);
+      __syncthreads();  // previous chunk consumed
+      for (int i = threadIdx.x; i < 3 * na * 64; i += 64 * GD_FW)
+        stage[i] = J.coords[(size_t)(3 * lb + (i >> 6)) * Rp + bk.rb + (i & 63)];
+      __syncthreads();
+      for (int l = 0; l < na; ++l) {
+        const float x = stage[(3 * l + 0) * 64 + lane];
+        const float y = stage[(3 * l + 1) * 64 + lane];
+        const float z = stage[(3 * l + 2) * 64 + lane];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (q < nm) {
+            const float4 t = J.mcs[(size_t)(m0 + q) * J.Ls + lb + l];
+            if (t.w != 0.0f) {
+              const float dx = x - t.x, dy = y - t.y, dz = z - t.z;
+              d[q] += __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < nm) {
+        const float2 row = J.mcsrow[m0 + q];
+        acc[7] += row.x * sqrtf(d[q] / row.y);
+      }
   }
 #pragma unroll
   for (int k = 0; k < 8; ++k) red[wave][k][lane] = acc[k];

@@ -165,7 +165,6 @@ static gd_status alloc_partials(gd_job *j, GdJob &g, int cap_rp) {
   const size_t nS = (size_t)g.nL * g.nG;
   a.add(&g.part_lin, (size_t)GD_NLIN * nS * cap_rp);
   a.add(&g.part_hpc, (size_t)g.L * g.nG * cap_rp);
-  a.add(&g.part_mcs, (size_t)std::max(1, g.M) * g.nL * cap_rp);
   return a.commit(&j->d_part);
 }
 
@@ -262,18 +261,32 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       o[7] = hb ? (float)(-(double)e->hdb0[lt][pt] * c) : 0.0f;
     }
   }
+  // Protein points in device order: grouped by class (cdc flag + 2 x class-2
+  // flag; stable), so the pair kernel runs one loop per class with the class
+  // terms resolved at compile time.  Point order only changes summation order.
+  std::vector<int> porder(P);
+  {
+    int nc[4] = {0, 0, 0, 0};
+    auto cls = [&](int p) { return (d->prt_cdc[p] == 1 ? 1 : 0) + (d->prt_c[p] == 2 ? 2 : 0); };
+    for (int p = 0; p < P; ++p) ++nc[cls(p)];
+    g.pcls[0] = 0;
+    for (int c = 0; c < 4; ++c) g.pcls[c + 1] = g.pcls[c] + nc[c];
+    int fill[4] = {g.pcls[0], g.pcls[1], g.pcls[2], g.pcls[3]};
+    for (int p = 0; p < P; ++p) porder[fill[cls(p)]++] = p;
+  }
   std::vector<float> pspv((size_t)(nty + 1) * P, 0.0f);  // last row: padding atoms (zero)
   std::vector<float4> pattr(P);
-  for (int p = 0; p < P; ++p) {
+  for (int q = 0; q < P; ++q) {
+    const int p = porder[q];
     const int c = d->prt_c[p];
     int fl = d->prt_t[p] | (d->prt_cdc[p] == 1 ? GD_PF_CDC : 0) | (c == 2 ? GD_PF_CLS2 : 0);
     float flf;
     std::memcpy(&flf, &fl, 4);
-    pattr[p] = make_float4(d->prt_ele[p], d->prt_hpp[p], flf, 0.0f);
+    pattr[q] = make_float4(d->prt_ele[p], d->prt_hpp[p], flf, 0.0f);
     if (c == 2) {
       const int res = d->prt_seq3r[p];
       for (int ti = 0; ti < nty; ++ti)
-        pspv[(size_t)ti * P + p] =
+        pspv[(size_t)ti * P + q] =
             (res >= 0 && res < d->psp_ncol) ? d->psp[(size_t)types[ti] * d->psp_ncol + res] : 0.0f;
     }
   }
@@ -328,9 +341,9 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       lig0[q] = make_float4(d->lig_x[q], d->lig_y[q], d->lig_z[q], 0.0f);
     }
   for (int i = 0; i < np; ++i)
-    for (int p = 0; p < P; ++p) {
-      const size_t q = (size_t)i * P + p;
-      pco[q] = make_float4(d->prt_x[q], d->prt_y[q], d->prt_z[q], 0.0f);
+    for (int q = 0; q < P; ++q) {
+      const size_t src = (size_t)i * P + porder[q];
+      pco[(size_t)i * P + q] = make_float4(d->prt_x[src], d->prt_y[src], d->prt_z[src], 0.0f);
     }
   // scalars
   g.lj0 = e->lj0, g.lj1 = e->lj1, g.el0 = e->el0, g.el1 = e->el1, g.a1 = e->a1, g.b1 = e->b1;
@@ -422,7 +435,7 @@ size_t footprint(const gd_job_desc *d) {
   default_tiling((int)L, (int)P, (int)R, lc, ptile);
   const size_t nL = (L + lc - 1) / lc, nP0 = (P + ptile - 1) / ptile;
   const size_t nG = (nP0 + GD_GROUP - 1) / GD_GROUP;
-  b += a(GD_NLIN * nL * nG * Rp * 4) + a(L * nG * Rp * 4) + a(std::max<size_t>(1, M) * nL * Rp * 4);
+  b += a(GD_NLIN * nL * nG * Rp * 4) + a(L * nG * Rp * 4);
   b += (Rp / 64) * nL * nG * sizeof(GdItem) + (Rp / 64) * sizeof(GdBlock) + 512;
   return b;
 }
@@ -487,7 +500,7 @@ gd_status sync_job_struct(gd_job *j) {
 gd_status launch_pair(int lc, const GdJob *d_jobs, const GdItem *d_items, int n_items,
                       hipStream_t s) {
   if (n_items <= 0) return GD_OK;
-  const dim3 grid(n_items), block(256);  // one 4-wave group per item
+  const dim3 grid(n_items), block(64 * GD_GROUP);  // one group per item
   switch (lc) {
     case 4:
       hipLaunchKernelGGL(gd_pair_kernel<4>, grid, block, 0, s, d_jobs, d_items, n_items);
@@ -694,7 +707,7 @@ size_t gd_job_footprint(const gd_job_desc *d) { return footprint(d); }
 size_t gd_job_device_bytes(const gd_job *j) {
   if (!j) return 0;
   const GdJob &g = j->h;
-  const size_t part = ((size_t)GD_NLIN * g.nL * g.nG + (size_t)g.L * g.nG + (size_t)g.M * g.nL) *
+  const size_t part = ((size_t)GD_NLIN * g.nL * g.nG + (size_t)g.L * g.nG) *
                       g.Rp * sizeof(float);
   return j->bytes + part;
 }
