@@ -25,6 +25,7 @@
 template <int LC>
 __global__ void gd_pair_kernel_wq(const GdJob *, const GdItem *, int, int *);
 __global__ void gd_transform_kernel(const GdJob *, const GdBlock *, int *);
+__global__ void gd_reduce_kernel(const GdJob *, const GdBlock *);
 __global__ void gd_finish_kernel(const GdJob *, const GdBlock *, int, int, const int *, int *);
 __global__ void gd_exchange_kernel(const GdJob *, const GdExch *);
 
@@ -133,6 +134,7 @@ gd_status batch_step(const BatchLists &B, int lc, int grid, int launch_step, con
   hipError_t e = launch_pair_wq(lc, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s);
   if (e != hipSuccess) return hip_fail(e, "pair kernel (queue) launch");
   if (tp && tp->end(s) != GD_OK) return GD_ERR_HIP;
+  hipLaunchKernelGGL(gd_reduce_kernel, dim3(B.n_blocks * GD_RK), dim3(256), 0, s, B.d_jobs, B.d_blocks);
   hipLaunchKernelGGL(gd_finish_kernel, dim3(B.n_blocks), dim3(1024), 0, s, B.d_jobs, B.d_blocks,
                      GD_MODE_STEP, launch_step, base, B.d_wq);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "finish kernel launch");
@@ -152,6 +154,7 @@ gd_status batch_init(const BatchLists &B, int lc, int grid, hipStream_t s) {
   if (e != hipSuccess) return hip_fail(e, "transform kernel launch");
   if ((e = launch_pair_wq(lc, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s)) != hipSuccess)
     return hip_fail(e, "pair kernel (queue) launch");
+  hipLaunchKernelGGL(gd_reduce_kernel, dim3(B.n_blocks * GD_RK), dim3(256), 0, s, B.d_jobs, B.d_blocks);
   hipLaunchKernelGGL(gd_finish_kernel, dim3(B.n_blocks), dim3(1024), 0, s, B.d_jobs, B.d_blocks,
                      GD_MODE_INIT, 0, (const int *)nullptr, B.d_wq);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "finish kernel launch");

@@ -20,7 +20,8 @@
 #define GD_PTAB_PT 32     // point-type stride of the table (31 used)
 #define GD_REC_W 17       // floats per accept record: step, movematrix[6], energy[10]
 #define GD_NLIN 6
-#define GD_GROUP 8        // protein tiles (waves) per pair-kernel work group, summed in LDS         // linear partial sums: vdw ele pmf psp hdb kde
+#define GD_GROUP 8        // protein tiles (waves) per pair-kernel work group, summed in LDS
+#define GD_RK 16          // reduction slices (work groups) per 64-replica block         // linear partial sums: vdw ele pmf psp hdb kde
 
 // point flag bits (pattr.z as int)
 #define GD_PF_TYPE 0x1f
@@ -76,6 +77,8 @@ struct GdJob {
   float *coords;          // [L][3][Rp] transformed ligand of the current proposal
   float *part_lin;        // [GD_NLIN][nL*nG][Rp]
   float *part_hpc;        // [L][nG][Rp]
+  float *part_mcs;        // [M][nL][Rp]
+  float *fin_part;        // [Rp/64][GD_RK][8][64]: per-slice sums of the reduce kernel
   float *rec;             // [Rp][rec_cap][GD_REC_W] accept records (optional)
   int *rec_n;             // [Rp]
   int rec_cap;

@@ -302,6 +302,25 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
     __builtin_amdgcn_sched_barrier(0);
   }
 
+  // MCS: this tile's rows b, b + nP, ..., restricted to the chunk atoms
+  {
+    const int M = J.M, nL = J.nL;
+    const GD_AS_CONST float4 *__restrict__ mcs = gd_c(J.mcs);
+    GD_AS_GLOBAL float *__restrict__ pm = gd_g(J.part_mcs);
+    for (int m = b; m < M; m += nP) {
+      float d = 0.0f;
+#pragma unroll
+      for (int j = 0; j < LC; ++j) {
+        const float4 t = mcs[(size_t)m * J.Ls + l0 + j];  // padding atoms: w == 0
+        if (t.w != 0.0f) {
+          const float dx = lx[j] - t.x, dy = ly[j] - t.y, dz = lz[j] - t.z;
+          d += __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        }
+      }
+      pm[((size_t)m * nL + it.a) * Rp + r] = d;
+    }
+  }
+
   // partial sums
   float eele = 0.0f;
 #pragma unroll
@@ -394,15 +413,22 @@ __device__ __forceinline__ void gd_rotation(const float m[6], float R[9]) {
   R[8] = c2 * c3;
 }
 
-__device__ __forceinline__ void gd_transform_atoms(const GdJob &J, int r, const float R[9],
+// The finish, transform and move code reads the job struct through the
+// constant address space (uniform fields become scalar loads) and per-replica
+// arrays through the global one; generic pointers made every access a flat
+// load followed by a full wait.
+typedef const GD_AS_CONST GdJob GdJobC;
+
+__device__ __forceinline__ void gd_transform_atoms(GdJobC &J, int r, const float R[9],
                                                    const float t[3], int wave, int nwave) {
   const int L = J.L, Rp = J.Rp;
-  const float4 *lc = J.lig0 + (size_t)J.ligi[r] * L;
+  const GD_AS_GLOBAL float4 *lc = gd_g(J.lig0) + (size_t)gd_g(J.ligi)[r] * L;
+  GD_AS_GLOBAL float *co = gd_g(J.coords);
   for (int l = wave; l < L; l += nwave) {
     const float4 a = lc[l];
-    J.coords[(size_t)(3 * l + 0) * Rp + r] = R[0] * a.x + R[1] * a.y + R[2] * a.z + t[0];
-    J.coords[(size_t)(3 * l + 1) * Rp + r] = R[3] * a.x + R[4] * a.y + R[5] * a.z + t[1];
-    J.coords[(size_t)(3 * l + 2) * Rp + r] = R[6] * a.x + R[7] * a.y + R[8] * a.z + t[2];
+    co[(size_t)(3 * l + 0) * Rp + r] = R[0] * a.x + R[1] * a.y + R[2] * a.z + t[0];
+    co[(size_t)(3 * l + 1) * Rp + r] = R[3] * a.x + R[4] * a.y + R[5] * a.z + t[1];
+    co[(size_t)(3 * l + 2) * Rp + r] = R[6] * a.x + R[7] * a.y + R[8] * a.z + t[2];
   }
 }
 
@@ -411,12 +437,13 @@ __global__ __launch_bounds__(256) void gd_transform_kernel(const GdJob *__restri
                                                             const GdBlock *__restrict__ blocks,
                                                             int *__restrict__ wq) {
   if (wq && blockIdx.x == 0 && threadIdx.x == 0) *wq = 0;  // next pair launch's queue
-  const GdBlock bk = blocks[blockIdx.x];
-  const GdJob &J = jobs[bk.job];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const GdBlock bk = gd_c(blocks)[blockIdx.x];
+  GdJobC &J = gd_c(jobs)[bk.job];
+  const int lane = threadIdx.x & 63, wave = gd_uniform((int)(threadIdx.x >> 6));
   const int r = bk.rb + lane;
+  const GD_AS_GLOBAL float *prop = gd_g(J.prop);
   float m[6];
-  for (int i = 0; i < 6; ++i) m[i] = J.prop[(size_t)i * J.Rp + r];
+  for (int i = 0; i < 6; ++i) m[i] = prop[(size_t)i * J.Rp + r];
   float R[9];
   gd_rotation(m, R);
   const float t[3] = {m[0] + J.center[0], m[1] + J.center[1], m[2] + J.center[2]};
@@ -426,7 +453,7 @@ __global__ __launch_bounds__(256) void gd_transform_kernel(const GdJob *__restri
 // ---------------------------------------------------------------------------
 // Move increments for step `step` (reference: move_scale * U[0,1),
 // kernel_cpp_montecarlo_soa.C:242-250; gaussian mode: move_scale * N(0,1)).
-__device__ __forceinline__ void gd_move(const GdJob &J, int r, uint32_t step, float inc[6],
+__device__ __forceinline__ void gd_move(GdJobC &J, int r, uint32_t step, float inc[6],
                                         float *u_acc) {
   uint32_t w[8];
   gd_step_draws(((uint64_t)J.seed_hi << 32) | J.seed_lo, J.job_id, (uint32_t)r, step, w);
@@ -448,31 +475,86 @@ __device__ __forceinline__ void gd_move(const GdJob &J, int r, uint32_t step, fl
 }
 
 // ---------------------------------------------------------------------------
-// Sum of n values at stride `st` (floats) with eight loads in flight; the
-// combination order depends only on n, so results are reproducible.
-__device__ __forceinline__ float gd_sum_strided(const float *__restrict__ p, int n, size_t st) {
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f, a5 = 0.0f, a6 = 0.0f, a7 = 0.0f;
-  int q = 0;
-  for (; q + 8 <= n; q += 8) {
-    const float *b = p + (size_t)q * st;
-    a0 += b[0];
-    a1 += b[st];
-    a2 += b[2 * st];
-    a3 += b[3 * st];
-    a4 += b[4 * st];
-    a5 += b[5 * st];
-    a6 += b[6 * st];
-    a7 += b[7 * st];
+// Sum of n values at stride `st` (floats) with eight loads in flight, the last
+// partial block too (indices clamped to n - 1, masked to zero); element q goes
+// to accumulator q mod 8, so the order depends only on n and is reproducible.
+__device__ __forceinline__ float gd_sum_strided(const GD_AS_GLOBAL float *__restrict__ p, int n,
+                                                size_t st) {
+  float a[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = 0.0f;
+  for (int q = 0; q < n; q += 8) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = p[(size_t)min(q + i, n - 1) * st];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] += q + i < n ? v[i] : 0.0f;
   }
-  for (; q < n; ++q) a0 += p[(size_t)q * st];
-  return ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
+  return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
+// ---------------------------------------------------------------------------
+// Reduce kernel: GD_RK work groups (4 waves) per 64-replica block; lanes are
+// replicas.  The 4 GD_RK waves of a block split the partial slots: each takes a
+// contiguous share of every linear term's slots, atoms l = w, w + 4 GD_RK, ...
+// (the hpc sum over tile groups, then the non-linear restraint) and MCS rows
+// m = w, w + 4 GD_RK, ... (the sum over chunks, then tcc sqrt(d/n)).  A group
+// sums its 4 waves in a fixed order and writes 8 values per replica; the
+// finish kernel sums the GD_RK slices in a fixed order.  The reduction reads
+// most of the step's partials, so it is spread over many groups: one group per
+// block kept too few loads in flight.
+__global__ __launch_bounds__(256) void gd_reduce_kernel(const GdJob *__restrict__ jobs,
+                                                         const GdBlock *__restrict__ blocks) {
+  __shared__ float red[4][8][64];
+  const int bi = blockIdx.x / GD_RK, k = blockIdx.x % GD_RK;
+  const GdBlock bk = gd_c(blocks)[bi];
+  GdJobC &J = gd_c(jobs)[bk.job];
+  const int lane = threadIdx.x & 63, wave = gd_uniform((int)(threadIdx.x >> 6));
+  const int r = bk.rb + lane;
+  const int Rp = J.Rp;
+  const int W = 4 * GD_RK, wi = 4 * k + wave;
+  const int nS = J.nL * J.nG;
+  const GD_AS_GLOBAL float *part_lin = gd_g(J.part_lin);
+  const GD_AS_GLOBAL float *part_hpc = gd_g(J.part_hpc);
+  const GD_AS_GLOBAL float *part_mcs = gd_g(J.part_mcs);
+  const GD_AS_CONST GdAtom *atoms = gd_c(J.atoms);
+  const GD_AS_CONST float2 *mcsrow = gd_c(J.mcsrow);
+  float acc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) acc[c] = 0.0f;
+  {
+    const int q0 = (int)(((long long)nS * wi) / W), q1 = (int)(((long long)nS * (wi + 1)) / W);
+#pragma unroll
+    for (int c = 0; c < GD_NLIN; ++c)
+      acc[c] = gd_sum_strided(part_lin + ((size_t)c * nS + q0) * Rp + r, q1 - q0, (size_t)Rp);
+  }
+  for (int l = wi; l < J.L; l += W) {  // hpc per atom (energy_prt_v1.C:132-133)
+    const float h = gd_sum_strided(part_hpc + (size_t)l * J.nG * Rp + r, J.nG, (size_t)Rp);
+    const GdAtom at = atoms[l];
+    const float t = (h - at.hpl0) / at.hpl1;
+    acc[6] += 0.5f * t * t - at.hpl2;
+  }
+  for (int m = wi; m < J.M; m += W) {  // MCS rows (energy_mcs_v1.C:5-29)
+    const float d = gd_sum_strided(part_mcs + (size_t)m * J.nL * Rp + r, J.nL, (size_t)Rp);
+    const float2 row = mcsrow[m];
+    acc[7] += row.x * sqrtf(d / row.y);
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) red[wave][c][lane] = acc[c];
+  __syncthreads();
+  if (wave == 0) {
+    GD_AS_GLOBAL float *fp = gd_g(J.fin_part) + ((size_t)(bk.rb / 64) * GD_RK + k) * 8 * 64;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      fp[c * 64 + lane] = ((red[0][c][lane] + red[1][c][lane]) + red[2][c][lane]) + red[3][c][lane];
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Finish kernel: 1024 threads = 16 waves per 64-replica block; lanes are
-// replicas, waves split the partial slots, atoms and MCS rows.
+// replicas.  Sums the reduce kernel's slices, assembles and weights the terms,
+// records, accepts, proposes the next move and transforms the atoms.
 #define GD_FW 16
-#define GD_FIN_AT 32  // atoms per LDS stage of the finish kernel's MCS pass
 __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict__ jobs,
                                                           const GdBlock *__restrict__ blocks,
                                                           int mode, int launch_step,
@@ -482,74 +564,29 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
   if (step_base) launch_step += *step_base;  // graph replays: base step in device memory
   __shared__ float red[GD_FW][8][64];
   __shared__ float xf[12][64];
-  __shared__ float stage[3 * GD_FIN_AT * 64];  // transformed atoms of the block, for MCS
-  const GdBlock bk = blocks[blockIdx.x];
-  const GdJob &J = jobs[bk.job];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const GdBlock bk = gd_c(blocks)[blockIdx.x];
+  GdJobC &J = gd_c(jobs)[bk.job];
+  const int lane = threadIdx.x & 63, wave = gd_uniform((int)(threadIdx.x >> 6));
   const int r = bk.rb + lane;
   const int Rp = J.Rp;
-  const int nS = J.nL * J.nG;
   const int step = launch_step - J.step0;  // this job's MC step
 
-  // (1) slot reductions: wave w takes the contiguous w-th share of the slots
-  // of each linear term, atoms l = w, w+16, ..., rows m = w, w+16, ...
-  float acc[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
+  // (1) the reduce kernel's slices: wave w reads slices w, w + 16, ...
   {
-    const int q0 = (int)(((long long)nS * wave) / GD_FW), q1 = (int)(((long long)nS * (wave + 1)) / GD_FW);
+    const GD_AS_GLOBAL float *fp = gd_g(J.fin_part) + (size_t)(bk.rb / 64) * GD_RK * 8 * 64;
+    float acc[8];
 #pragma unroll
-    for (int k = 0; k < GD_NLIN; ++k)
-      acc[k] = gd_sum_strided(J.part_lin + ((size_t)k * nS + q0) * Rp + r, q1 - q0, (size_t)Rp);
+    for (int c = 0; c < 8; ++c) acc[c] = 0.0f;
+    for (int k = wave; k < GD_RK; k += GD_FW)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[c] += fp[((size_t)k * 8 + c) * 64 + lane];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) red[wave][c][lane] = acc[c];
   }
-  for (int l = wave; l < J.L; l += GD_FW) {  // hpc per atom (energy_prt_v1.C:132-133)
-    const float h = gd_sum_strided(J.part_hpc + (size_t)l * J.nG * Rp + r, J.nG, (size_t)Rp);
-    const GdAtom at = J.atoms[l];
-    const float t = (h - at.hpl0) / at.hpl1;
-    acc[6] += 0.5f * t * t - at.hpl2;
-  }
-  // MCS rows (energy_mcs_v1.C:5-29) from the transformed atoms of this step's
-  // proposal, staged in LDS 32 atoms at a time by the whole block; wave w takes
-  // rows 4w .. 4w + 3, 64 + 4w .. 64 + 4w + 3, ...; a row's squared distances
-  // are summed in atom order
-  for (int mb = 0; mb < J.M; mb += 4 * GD_FW) {
-    const int m0 = mb + 4 * wave;
-    const int nm = max(0, min(4, J.M - m0));
-    float d[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int lb = 0; lb < J.L; lb += GD_FIN_AT) {
-      const int na = min(GD_FIN_AT, J.L - lb);
-      __syncthreads();  // previous chunk consumed
-      for (int i = threadIdx.x; i < 3 * na * 64; i += 64 * GD_FW)
-        stage[i] = J.coords[(size_t)(3 * lb + (i >> 6)) * Rp + bk.rb + (i & 63)];
-      __syncthreads();
-      for (int l = 0; l < na; ++l) {
-        const float x = stage[(3 * l + 0) * 64 + lane];
-        const float y = stage[(3 * l + 1) * 64 + lane];
-        const float z = stage[(3 * l + 2) * 64 + lane];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (q < nm) {
-            const float4 t = J.mcs[(size_t)(m0 + q) * J.Ls + lb + l];
-            if (t.w != 0.0f) {
-              const float dx = x - t.x, dy = y - t.y, dz = z - t.z;
-              d[q] += __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-            }
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (q < nm) {
-        const float2 row = J.mcsrow[m0 + q];
-        acc[7] += row.x * sqrtf(d[q] / row.y);
-      }
-  }
-#pragma unroll
-  for (int k = 0; k < 8; ++k) red[wave][k][lane] = acc[k];
   __syncthreads();
 
   if (wave == 0) {
+    GD_AS_GLOBAL float *prop = gd_g(J.prop), *en = gd_g(J.en), *mm = gd_g(J.mm);
     float s[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -560,7 +597,7 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
     }
     float m[6];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) m[i] = J.prop[(size_t)i * Rp + r];
+    for (int i = 0; i < 6; ++i) m[i] = prop[(size_t)i * Rp + r];
     // e_s assembly (kernel_cpp_montecarlo_soa.C:318-324), KDE/MCS/DST (:328-359)
     float e[10];
     e[0] = s[0];
@@ -597,19 +634,22 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
     const bool live = r < J.R;
     if (mode == GD_MODE_EVAL) {
       if (live) {
-        for (int i = 0; i < 10; ++i) J.ev_terms[(size_t)r * 10 + i] = e[i];
-        for (int i = 0; i < 9; ++i) J.ev_raw[(size_t)r * 9 + i] = raw[i];
+        GD_AS_GLOBAL float *evt = gd_g(J.ev_terms), *evr = gd_g(J.ev_raw);
+        for (int i = 0; i < 10; ++i) evt[(size_t)r * 10 + i] = e[i];
+        for (int i = 0; i < 9; ++i) evr[(size_t)r * 9 + i] = raw[i];
       }
     } else {
+      GD_AS_GLOBAL int *acc_r = gd_g(J.acc);
       // record the pre-move state if the previous step was accepted (:224-231)
-      if (mode == GD_MODE_STEP && J.record && live && J.acc[r]) {
-        const int n = J.rec_n[r];
+      if (mode == GD_MODE_STEP && J.record && live && acc_r[r]) {
+        GD_AS_GLOBAL int *rec_n = gd_g(J.rec_n);
+        const int n = rec_n[r];
         if (n < J.rec_cap) {
-          float *o = J.rec + ((size_t)r * J.rec_cap + n) * GD_REC_W;
+          GD_AS_GLOBAL float *o = gd_g(J.rec) + ((size_t)r * J.rec_cap + n) * GD_REC_W;
           o[0] = (float)step;
-          for (int i = 0; i < 6; ++i) o[1 + i] = J.mm[(size_t)i * Rp + r];
-          for (int i = 0; i < 10; ++i) o[7 + i] = J.en[(size_t)i * Rp + r];
-          J.rec_n[r] = n + 1;
+          for (int i = 0; i < 6; ++i) o[1 + i] = mm[(size_t)i * Rp + r];
+          for (int i = 0; i < 10; ++i) o[7 + i] = en[(size_t)i * Rp + r];
+          rec_n[r] = n + 1;
         }
       }
       // Metropolis (:382-392); the initial pass draws u = 0
@@ -618,15 +658,16 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
         float inc[6];
         gd_move(J, r, (uint32_t)step, inc, &u);
       }
-      const float x = (tot - J.en[(size_t)9 * Rp + r]) * J.minus_beta[J.tmpi[r]];
+      const float x = (tot - en[(size_t)9 * Rp + r]) * gd_g(J.minus_beta)[gd_g(J.tmpi)[r]];
       const bool a = (u == 0.0f) ? (x >= GD_EXP_UNDERFLOW) : (u < expf(x));
       if (live) {
-        J.acc[r] = a;
+        acc_r[r] = a;
         if (a) {
-          for (int i = 0; i < 10; ++i) J.en[(size_t)i * Rp + r] = e[i];
-          for (int i = 0; i < 6; ++i) J.mm[(size_t)i * Rp + r] = m[i];
-          if (mode == GD_MODE_STEP) J.nacc[r] += 1;
-          J.best[r] = fminf(J.best[r], tot);
+          for (int i = 0; i < 10; ++i) en[(size_t)i * Rp + r] = e[i];
+          for (int i = 0; i < 6; ++i) mm[(size_t)i * Rp + r] = m[i];
+          if (mode == GD_MODE_STEP) gd_g(J.nacc)[r] += 1;
+          GD_AS_GLOBAL float *best = gd_g(J.best);
+          best[r] = fminf(best[r], tot);
         }
       }
       // propose and transform the next step's move
@@ -635,8 +676,8 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
       gd_move(J, r, next, inc, &un);
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
-        const float v = inc[i] + J.mm[(size_t)i * Rp + r];
-        J.prop[(size_t)i * Rp + r] = v;
+        const float v = inc[i] + mm[(size_t)i * Rp + r];
+        prop[(size_t)i * Rp + r] = v;
         m[i] = v;
       }
       float R[9];

@@ -30,6 +30,7 @@
 template <int LC>
 __global__ void gd_pair_kernel(const GdJob *, const GdItem *, int);
 __global__ void gd_transform_kernel(const GdJob *, const GdBlock *, int *);
+__global__ void gd_reduce_kernel(const GdJob *, const GdBlock *);
 __global__ void gd_finish_kernel(const GdJob *, const GdBlock *, int, int, const int *, int *);
 __global__ void gd_exchange_kernel(const GdJob *, const GdExch *);
 
@@ -165,6 +166,8 @@ static gd_status alloc_partials(gd_job *j, GdJob &g, int cap_rp) {
   const size_t nS = (size_t)g.nL * g.nG;
   a.add(&g.part_lin, (size_t)GD_NLIN * nS * cap_rp);
   a.add(&g.part_hpc, (size_t)g.L * g.nG * cap_rp);
+  a.add(&g.part_mcs, (size_t)std::max(1, g.M) * g.nL * cap_rp);
+  a.add(&g.fin_part, (size_t)(cap_rp / 64) * GD_RK * 8 * 64);
   return a.commit(&j->d_part);
 }
 
@@ -435,7 +438,8 @@ size_t footprint(const gd_job_desc *d) {
   default_tiling((int)L, (int)P, (int)R, lc, ptile);
   const size_t nL = (L + lc - 1) / lc, nP0 = (P + ptile - 1) / ptile;
   const size_t nG = (nP0 + GD_GROUP - 1) / GD_GROUP;
-  b += a(GD_NLIN * nL * nG * Rp * 4) + a(L * nG * Rp * 4);
+  b += a(GD_NLIN * nL * nG * Rp * 4) + a(L * nG * Rp * 4) + a(std::max<size_t>(1, M) * nL * Rp * 4) +
+       a((Rp / 64) * GD_RK * 8 * 64 * 4);
   b += (Rp / 64) * nL * nG * sizeof(GdItem) + (Rp / 64) * sizeof(GdBlock) + 512;
   return b;
 }
@@ -521,6 +525,7 @@ gd_status launch_pair(int lc, const GdJob *d_jobs, const GdItem *d_items, int n_
 gd_status launch_finish(const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks, int mode,
                         int step, hipStream_t s) {
   if (n_blocks <= 0) return GD_OK;
+  hipLaunchKernelGGL(gd_reduce_kernel, dim3(n_blocks * GD_RK), dim3(256), 0, s, d_jobs, d_blocks);
   hipLaunchKernelGGL(gd_finish_kernel, dim3(n_blocks), dim3(1024), 0, s, d_jobs, d_blocks, mode, step,
                      (const int *)nullptr, (int *)nullptr);
   hipError_t e = hipGetLastError();
@@ -707,7 +712,8 @@ size_t gd_job_footprint(const gd_job_desc *d) { return footprint(d); }
 size_t gd_job_device_bytes(const gd_job *j) {
   if (!j) return 0;
   const GdJob &g = j->h;
-  const size_t part = ((size_t)GD_NLIN * g.nL * g.nG + (size_t)g.L * g.nG) *
+  const size_t part = ((size_t)GD_NLIN * g.nL * g.nG + (size_t)g.L * g.nG + (size_t)g.M * g.nL +
+                       (size_t)GD_RK * 8) *
                       g.Rp * sizeof(float);
   return j->bytes + part;
 }
