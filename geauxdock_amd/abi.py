@@ -140,6 +140,7 @@ EXTRA_PROTOTYPES = {
     "gd_host_job_diag": (C.c_int, [_vp, _ip, _fp, C.c_int, _ip, C.c_int]),
     "gd_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int,
                          C.POINTER(C.c_uint32)]),
+    "gd_far_r2": (C.c_float, [C.c_float, C.c_float]),
 }
 
 

@@ -16,7 +16,7 @@
 #include <stdint.h>
 
 #define GD_WAVE 64
-#define GD_PTAB_W 8       // floats per (ligand type, point type) entry
+#define GD_PTAB_W 12      // floats per (ligand type, point type) entry (9 used)
 #define GD_PTAB_PT 32     // point-type stride of the table (31 used)
 #define GD_REC_W 17       // floats per accept record: step, movematrix[6], energy[10]
 #define GD_NLIN 6
@@ -59,7 +59,8 @@ struct GdJob {
   const float4 *pattr;    // [P] (ele, hpp, flags-as-int, 0)
   const float *ptab;      // [n_types + 1][GD_PTAB_PT][GD_PTAB_W] (last block: padding atoms):
                           // p1a, -p2a, PMF exponent B and C,
-                          // pmf1, psp r^2 threshold (sign bit: H-bond active), H-bond c, -hdb0 c
+                          // pmf1, psp r^2 threshold (sign bit: H-bond active), H-bond c, -hdb0 c,
+                          // far-field r^2 threshold
   const float *pspv;      // [n_types][P] psp value of class-2 points
   const float4 *kde;      // [K] sorted by type
   const float4 *mcs;      // [M][Ls] (x, y, z, valid)

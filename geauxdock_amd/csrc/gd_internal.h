@@ -38,6 +38,7 @@ gd_status sync_job_struct(gd_job *j);
 gd_status reset_replicas(gd_job *j);
 void set_mc(GdJob &g, const gd_mc_param *mc);
 gd_status read_out(gd_job *j, gd_replica_out *out);
+float far_r2_threshold(double pmf0, double el1);
 gd_status launch_pair(int lc, const GdJob *d_jobs, const GdItem *d_items, int n_items, hipStream_t s);
 gd_status launch_finish(const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks, int mode,
                         int step, hipStream_t s);

@@ -42,8 +42,6 @@ __device__ __forceinline__ float gd_exp2(float x) { return __builtin_amdgcn_exp2
 #define GD_HPB (4.5f / 81.0f / 81.0f)
 #define GD_HPC (2.5f / 81.0f / 81.0f / 81.0f)
 #define GD_HPD (0.5f / 81.0f / 81.0f / 81.0f / 81.0f)
-// PMF exponents below this give exp2(x) < 2^-25: 1 + exp2(x) rounds to 1
-#define GD_PMF_FAR (-25.0f)
 // 1 / sqrt(log2(e) / 2) = sqrt(2 ln 2): H-bond sums carry the factor sqrt(log2(e) / 2)
 #define GD_HDB_RC 1.17741002251547469f
 
@@ -99,11 +97,13 @@ __device__ __forceinline__ void gd_pair_atoms(
     const GD_AS_CONST float *__restrict__ prm =
         (const GD_AS_CONST float *)(ptab + (toff[j] + poff));
     const float p1a = prm[0], np2a = prm[1], pB = prm[2], pC = prm[3];
-    const float pmf1 = prm[4], pthr = prm[5], hdb1c = prm[6], nhdb0c = prm[7];
+    const float pmf1 = prm[4], pthr = prm[5], hdb1c = prm[6], nhdb0c = prm[7], r2far = prm[8];
     // the sign bit of the psp threshold flags an active H-bond pair; the asm
     // keeps the test on the scalar unit (as a float compare it became a VALU op)
     int hflag;
     asm("" : "=s"(hflag) : "0"(__float_as_int(pthr)));
+    // the near-field coefficients are loaded with the rest of the row (one wait)
+    asm volatile("" ::"s"(pB), "s"(pC));
     const float dx = lx[j] - c.x, dy = ly[j] - c.y, dz = lz[j] - c.z;
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     const float ri = gd_rsq(r2);
@@ -132,21 +132,20 @@ __device__ __forceinline__ void gd_pair_atoms(
       const float den = __builtin_fmaf(p1a, __builtin_fmaf(k.lj01, r2, k.lj0), r9);
       evdw = __builtin_fmaf(num, gd_rcp(den), evdw);
     }
-    // contact PMF (:88-91): exponent (6 - r/2)(r - pmf0) log2(e)
-    // = -log2(e)/2 r^2 + pB r + pC
-    const float arg = __builtin_fmaf(pB, rr, __builtin_fmaf(pmf_a, r2, pC));
-    // Far field, decided for the whole wave: below arg = -25, exp2(arg) <
-    // 2^-25 and 1 + exp2(arg) rounds to 1, so the PMF term is exactly pmf1;
-    // at s = el1 r >= 1 the electrostatic factor is 1/s (scaled: 1/r).
-    const bool ele_near = rr < k.inv_el1;
-    const bool near = (arg >= GD_PMF_FAR) | ele_near;
+    // Far field, decided for the whole wave on one compare: beyond the row's
+    // r^2 threshold the PMF exponent is below -26, so 1 + exp2 of it rounds to 1
+    // and the PMF term is exactly pmf1, and s = el1 r >= 1, so the electrostatic
+    // factor is 1/s (scaled: 1/r)
     float g = ri;
-    if (__ballot(near) != 0) {
+    if (__ballot(r2 < r2far) != 0) {
       asm volatile("");
+      // contact PMF (:88-91): exponent (6 - r/2)(r - pmf0) log2(e)
+      // = -log2(e)/2 r^2 + pB r + pC
+      const float arg = __builtin_fmaf(pB, rr, __builtin_fmaf(pmf_a, r2, pC));
       // distance-dependent electrostatics (:72-78), charge applied per atom:
       // el1 g = poly(r) for s = el1 r < 1, else 1/r
       const float poly = __builtin_fmaf(__builtin_fmaf(k.e3, rr, k.e2), r2, k.e0);
-      g = ele_near ? poly : ri;
+      g = rr < k.inv_el1 ? poly : ri;
       epmf = __builtin_fmaf(pmf1, gd_rcp(1.0f + gd_exp2(arg)), epmf);
     } else {
       epmf += pmf1;

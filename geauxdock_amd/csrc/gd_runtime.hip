@@ -134,6 +134,25 @@ static float sqrt_threshold(float pmf0) {
   return x;
 }
 
+// Far-field threshold on r^2 for one (ligand type, point type) pair: from it on
+// the PMF exponent (6 - r/2)(r - pmf0) log2(e) stays below -26 (so 1 + exp2 of it
+// rounds to 1 in FP32 even with the kernel's rounding of the exponent) and
+// el1 r >= 1 (the electrostatic factor is 1/(el1 r)); both hold for every larger r
+// because the exponent decreases beyond r = 6 + pmf0/2.  Rounded up.
+float far_r2_threshold(double pmf0, double el1) {
+  if (!(el1 > 0.0)) return std::numeric_limits<float>::infinity();
+  const double log2e = 1.44269504088896340736;
+  // -log2(e)/2 r^2 + log2(e)(6 + pmf0/2) r - 6 log2(e) pmf0 + 26 = 0, larger root
+  const double qa = -0.5 * log2e, qb = log2e * (6.0 + 0.5 * pmf0), qc = -6.0 * log2e * pmf0 + 26.0;
+  const double disc = qb * qb - 4.0 * qa * qc;
+  double r = disc > 0.0 ? (-qb - std::sqrt(disc)) / (2.0 * qa) : 6.0 + 0.5 * pmf0;
+  r = std::max(r, 6.0 + 0.5 * pmf0) * (1.0 + 1e-6);
+  while (log2e * (6.0 - 0.5 * r) * (r - pmf0) > -26.0) r *= 1.0 + 1e-6;
+  r = std::max(r, (1.0 / el1) * (1.0 + 1e-6));
+  const float f = (float)(r * r);
+  return (double)f >= r * r ? f : std::nextafter(f, std::numeric_limits<float>::infinity());
+}
+
 // ---------------------------------------------------------------------------
 // Tiling: ligand chunk size lc (kernel template), protein tile ptile.  The
 // default aims for ~4 waves per SIMD over the whole launch; it depends only
@@ -262,6 +281,7 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       const double c = (double)e->hdb1[lt][pt] * hdb_c;
       o[6] = hb ? (float)c : 0.0f;
       o[7] = hb ? (float)(-(double)e->hdb0[lt][pt] * c) : 0.0f;
+      o[8] = far_r2_threshold(pmf0, e->el1);
     }
   }
   // Protein points in device order: grouped by class (cdc flag + 2 x class-2
@@ -643,6 +663,9 @@ gd_status gd_device_info(char *name, int name_len, int *n_cu_out, int *clock_khz
 
 const char *gd_last_error(void) { return last_error(); }
 const char *gd_version(void) { return "geauxdock_amd 0.1 (gfx950)"; }
+
+// The pair kernel's far-field r^2 threshold for (pmf0, el1) (diagnostics, tests).
+float gd_far_r2(float pmf0, float el1) { return gdr::far_r2_threshold(pmf0, el1); }
 
 void gd_philox(const uint32_t ctr[4], const uint32_t key[2], int rounds, uint32_t out[4]) {
   const gd_u32x4 r = gd_philox4x32(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], rounds);

@@ -337,3 +337,19 @@ def test_cpu_dock_entry_point(golden_job, oracle):
     assert L.gd_cpu_dock(C.byref(golden_job.desc), C.byref(mc), C.byref(out.c)) == 0
     ref, _, _ = oracle.dock(golden_job, mc, threads=2)
     assert np.array_equal(out.energy, ref.energy) and np.array_equal(out.n_accept, ref.n_accept)
+
+
+@pytest.mark.parametrize("el1", [0.5, 0.1, 1.0])
+def test_far_field_threshold(el1):
+    """The pair kernel's far-field r^2 threshold (library, FP64 host code): from it
+    on the PMF exponent (6 - r/2)(r - pmf0) log2(e) stays <= -26 and el1 r >= 1, so
+    the shortcut (PMF term = pmf1, electrostatic factor 1/r) is exact up to FP32
+    rounding; and it is tight (2% closer violates one of the two)."""
+    log2e = 1.0 / math.log(2.0)
+    for pmf0 in np.linspace(0.0, 8.0, 17):
+        r = math.sqrt(g.lib().gd_far_r2(float(pmf0), el1))
+        for rp in r * (1.0 + np.linspace(0.0, 4.0, 400)):
+            assert log2e * (6.0 - rp / 2.0) * (rp - pmf0) <= -26.0 + 1e-9, (pmf0, rp)
+            assert el1 * rp >= 1.0
+        rs = 0.98 * r
+        assert log2e * (6.0 - rs / 2.0) * (rs - pmf0) > -26.0 or el1 * rs < 1.0
