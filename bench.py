@@ -10,7 +10,7 @@ replicas; Gaussian moves N(0, 0.2 A) / N(0, 3 deg); temperature exchange every
 10,000-step job.  FP32 arithmetic, synthetic data (seeded generator).
 
 Prints one JSON line (value = replica-steps/s).  The roofline object covers
-the dominant kernel (gd_pair_kernel<8>): algorithmic FP32 flops per launch /
+the dominant kernel (gd_pair_kernel<LC>, LC ligand atoms per lane): algorithmic FP32 flops per launch /
 its mean launch time from HIP events recorded around every launch on the
 job's stream.  cpu_baseline times the CPU oracle (a restatement of the
 reference CPU backend) on a bounded sample of the same workload.
@@ -321,7 +321,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic,
-            "kernel": "gd_pair_kernel<8>",
+            "kernel": "gd_pair_kernel<%d>" % dev.tiling()[0],
             "flops_per_launch": flops_step,
             "prt_pairs_per_launch": prt_pairs,
             "mean_launch_ms": pair_ms,

@@ -82,6 +82,8 @@ struct GdPairConst {
   float inv_el1;
 };
 
+#define GD_ROW 9  // force-field row entries used by the pair loop
+
 // The LC chunk atoms against one point.  CLS: bit 0 = cdc point (hydrophobic
 // term), bit 1 = class-2 point (psp term); the caller dispatches on the point's
 // flags once, so the atoms carry no per-pair class tests.
@@ -92,18 +94,33 @@ __device__ __forceinline__ void gd_pair_atoms(
     const float4 c, const float pele, const float phpp, const float pmf_a, const float (&lx)[LC],
     const float (&ly)[LC], const float (&lz)[LC], const unsigned (&toff)[LC], float (&ele)[LC],
     float (&hpc)[LC], float &evdw, float &epmf, float &epsp, float &ehdb) {
+  // Force-field rows are software-pipelined: the row of atom j + 1 is loaded
+  // while atom j is evaluated.  Scalar loads complete out of order, so every
+  // wait is for all of them; the asm with a memory clobber places the wait for
+  // row j before the load of row j + 1 is issued.
+  float rw[GD_ROW];
+  {
+    const GD_AS_CONST float *__restrict__ prm = (const GD_AS_CONST float *)(ptab + (toff[0] + poff));
+#pragma unroll
+    for (int i = 0; i < GD_ROW; ++i) rw[i] = prm[i];
+  }
 #pragma unroll
   for (int j = 0; j < LC; ++j) {
-    const GD_AS_CONST float *__restrict__ prm =
-        (const GD_AS_CONST float *)(ptab + (toff[j] + poff));
-    const float p1a = prm[0], np2a = prm[1], pB = prm[2], pC = prm[3];
-    const float pmf1 = prm[4], pthr = prm[5], hdb1c = prm[6], nhdb0c = prm[7], r2far = prm[8];
+    asm volatile("" ::"s"(rw[0]), "s"(rw[1]), "s"(rw[2]), "s"(rw[3]), "s"(rw[4]), "s"(rw[5]),
+                 "s"(rw[6]), "s"(rw[7]), "s"(rw[8])
+                 : "memory");
+    const float p1a = rw[0], np2a = rw[1], pB = rw[2], pC = rw[3];
+    const float pmf1 = rw[4], pthr = rw[5], hdb1c = rw[6], nhdb0c = rw[7], r2far = rw[8];
+    if (j + 1 < LC) {
+      const GD_AS_CONST float *__restrict__ prm =
+          (const GD_AS_CONST float *)(ptab + (toff[j + 1 < LC ? j + 1 : j] + poff));
+#pragma unroll
+      for (int i = 0; i < GD_ROW; ++i) rw[i] = prm[i];
+    }
     // the sign bit of the psp threshold flags an active H-bond pair; the asm
     // keeps the test on the scalar unit (as a float compare it became a VALU op)
     int hflag;
     asm("" : "=s"(hflag) : "0"(__float_as_int(pthr)));
-    // the near-field coefficients are loaded with the rest of the row (one wait)
-    asm volatile("" ::"s"(pB), "s"(pC));
     const float dx = lx[j] - c.x, dy = ly[j] - c.y, dz = lz[j] - c.z;
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     const float ri = gd_rsq(r2);
