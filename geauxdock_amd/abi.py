@@ -17,6 +17,7 @@ STATUS_NAMES = {0: "GD_OK", 1: "GD_ERR_ARG", 2: "GD_ERR_HIP", 3: "GD_ERR_SIZE", 
                 5: "GD_ERR_IO", 6: "GD_ERR_NOMEM", 7: "GD_ERR_STATE", 8: "GD_ERR_BUSY",
                 9: "GD_ERR_NODEV"}
 GD_MOVE_REF_UNIFORM, GD_MOVE_GAUSSIAN = 0, 1
+GD_EXCH_TEMP, GD_EXCH_LIG, GD_EXCH_PRT = 1, 2, 4
 
 _f = C.c_float
 _fp = C.POINTER(C.c_float)
@@ -57,7 +58,7 @@ class gd_mc_param(C.Structure):
     _fields_ = [
         ("steps_total", C.c_int), ("steps_per_dump", C.c_int), ("steps_per_exchange", C.c_int),
         ("move_mode", C.c_int), ("move_scale", _f * 6), ("seed", C.c_uint64),
-        ("job_id", C.c_uint32), ("record", C.c_int),
+        ("job_id", C.c_uint32), ("record", C.c_int), ("exchange_axes", C.c_int),
     ]
 
 
@@ -65,6 +66,7 @@ class gd_replica_out(C.Structure):
     _fields_ = [
         ("movematrix", _fp), ("energy", _fp), ("idx_tmp", _ip), ("n_accept", _ip),
         ("n_exchange", _ip), ("best_energy", _fp), ("is_accept", _ip),
+        ("n_swap_lig", _ip), ("n_swap_prt", _ip),
     ]
 
 

@@ -59,6 +59,8 @@ struct BatchLists {
   int *d_wq = nullptr;       // work-queue counter
   int *d_base = nullptr;     // base launch step for graph replays
   GdExch *d_ex = nullptr;    // exchange entries (scratch, per launch)
+  int *d_swr = nullptr;      // [n_jobs] conformer-swap round per job, -1 = none (scratch)
+  std::vector<int> axes;     // [n_jobs] GD_EXCH_* bits that can act (axis bit and > 1 entries)
   int n_items = 0, n_blocks = 0, n_jobs = 0, ex_cap = 0;
   void *mem = nullptr;
   ~BatchLists() {
@@ -69,9 +71,14 @@ struct BatchLists {
     std::vector<GdJob> js;
     std::vector<GdItem> items;
     std::vector<GdBlock> blocks;
+    axes.clear();
     for (int k = 0; k < (int)which.size(); ++k) {
       gd_job *j = jobs[which[k]];
       js.push_back(j->h);
+      const GdJob &h = j->h;
+      axes.push_back(((h.xaxes & GD_EXCH_TEMP) && h.n_tmp > 1 ? GD_EXCH_TEMP : 0) |
+                     ((h.xaxes & GD_EXCH_LIG) && h.n_lig > 1 ? GD_EXCH_LIG : 0) |
+                     ((h.xaxes & GD_EXCH_PRT) && h.n_prt > 1 ? GD_EXCH_PRT : 0));
       // job-local items in protein-tile-major order: waves that start together
       // share the same protein tile (scalar-cache and L2 reuse)
       for (const GdItem &it : j->items) items.push_back({k, it.rb, it.a, it.b});
@@ -90,16 +97,18 @@ struct BatchLists {
     const size_t bi = (items.size() * sizeof(GdItem) + 255) / 256 * 256;
     const size_t bb = (blocks.size() * sizeof(GdBlock) + 255) / 256 * 256;
     const size_t be = ((size_t)ex_cap * sizeof(GdExch) + 255) / 256 * 256;
+    const size_t bs = ((size_t)ex_cap * sizeof(int) + 255) / 256 * 256;
     if (mem) (void)hipFree(mem);
     mem = nullptr;
-    hipError_t e = hipMalloc(&mem, bj + bi + bb + be + 512);
+    hipError_t e = hipMalloc(&mem, bj + bi + bb + be + bs + 512);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc batch");
     char *p = (char *)mem;
     d_jobs = (GdJob *)p;
     d_items = (GdItem *)(p + bj);
     d_blocks = (GdBlock *)(p + bj + bi);
     d_ex = (GdExch *)(p + bj + bi + bb);
-    d_wq = (int *)(p + bj + bi + bb + be);
+    d_swr = (int *)(p + bj + bi + bb + be);
+    d_wq = (int *)(p + bj + bi + bb + be + bs);
     d_base = d_wq + 64;
     if ((e = hipMemcpyAsync(d_jobs, js.data(), js.size() * sizeof(GdJob), hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(GdItem), hipMemcpyHostToDevice, s)) != hipSuccess ||
@@ -127,9 +136,65 @@ int pair_waves_per_cu(int lc) { return lc == 4 ? 32 : lc == 8 ? 20 : 8; }
 constexpr int kSharedLc = 4;
 int shared_ptile(int P) { return std::min(P, 64); }
 
-// One MC step of every job in the lists (pair + finish), optional exchange.
+// Work of one exchange point: temperature-exchange entries and, per job of the
+// lists, the round of its conformer swaps (-1 = none).
+struct ExchPoint {
+  std::vector<GdExch> ex;
+  std::vector<int> swr;
+  int axes = 0;  // swap axes to launch (GD_EXCH_LIG / GD_EXCH_PRT)
+  bool empty() const { return ex.empty() && axes == 0; }
+  // round[q] >= 0 marks list job q as being at an exchange point
+  void set(const BatchLists &B, const std::vector<int> &round) {
+    ex.clear();
+    swr.assign(B.n_jobs, -1);
+    axes = 0;
+    for (int q = 0; q < B.n_jobs; ++q) {
+      if (round[q] < 0) continue;
+      if (B.axes[q] & GD_EXCH_TEMP) ex.push_back({q, round[q]});
+      if (B.axes[q] & (GD_EXCH_LIG | GD_EXCH_PRT)) {
+        swr[q] = round[q];
+        axes |= B.axes[q] & (GD_EXCH_LIG | GD_EXCH_PRT);
+      }
+    }
+  }
+};
+
+// Exchange kernels of one exchange point, tables already on the device:
+// temperature exchange (n_ex entries of B.d_ex), then the swap sub-steps.
+gd_status exchange_launch(const BatchLists &B, int lc, int grid, int n_ex, int axes, int launch_step,
+                          const int *base, hipStream_t s) {
+  if (n_ex > 0) {
+    hipLaunchKernelGGL(gd_exchange_kernel, dim3(n_ex), dim3(256), 0, s, B.d_jobs, B.d_ex);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "exchange kernel launch");
+  }
+  const std::function<gd_status()> pair = [&]() {
+    hipError_t e = launch_pair_wq(lc, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s);
+    return e == hipSuccess ? GD_OK : hip_fail(e, "pair kernel (queue) launch");
+  };
+  for (int axis = 0; axis < 2; ++axis)
+    if (axes & (GD_EXCH_LIG << axis)) {
+      gd_status st = launch_swap(axis, B.d_jobs, B.d_blocks, B.n_blocks, B.d_swr, launch_step, base,
+                                 B.d_wq, pair, s);
+      if (st != GD_OK) return st;
+    }
+  return GD_OK;
+}
+
+// Upload the tables of an exchange point (the caller synchronises before the
+// tables are rewritten).
+gd_status exchange_upload(const BatchLists &B, const ExchPoint &xp, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  if (!xp.ex.empty())
+    e = hipMemcpyAsync(B.d_ex, xp.ex.data(), xp.ex.size() * sizeof(GdExch), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && xp.axes)
+    e = hipMemcpyAsync(B.d_swr, xp.swr.data(), xp.swr.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  return e == hipSuccess ? GD_OK : hip_fail(e, "hipMemcpyAsync exchange tables");
+}
+
+// One MC step of every job in the lists (pair + finish), optional exchange point.
 gd_status batch_step(const BatchLists &B, int lc, int grid, int launch_step, const int *base,
-                     const std::vector<GdExch> *ex, hipStream_t s, KTimer *tp) {
+                     const ExchPoint *xp, hipStream_t s, KTimer *tp) {
   if (tp && tp->begin(s) != GD_OK) return GD_ERR_HIP;
   hipError_t e = launch_pair_wq(lc, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s);
   if (e != hipSuccess) return hip_fail(e, "pair kernel (queue) launch");
@@ -138,11 +203,10 @@ gd_status batch_step(const BatchLists &B, int lc, int grid, int launch_step, con
   hipLaunchKernelGGL(gd_finish_kernel, dim3(B.n_blocks), dim3(1024), 0, s, B.d_jobs, B.d_blocks,
                      GD_MODE_STEP, launch_step, base, B.d_wq);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "finish kernel launch");
-  if (ex && !ex->empty()) {
-    e = hipMemcpyAsync(B.d_ex, ex->data(), ex->size() * sizeof(GdExch), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync exch");
-    hipLaunchKernelGGL(gd_exchange_kernel, dim3((int)ex->size()), dim3(256), 0, s, B.d_jobs, B.d_ex);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "exchange kernel launch");
+  if (xp && !xp->empty()) {
+    gd_status st = exchange_upload(B, *xp, s);
+    if (st != GD_OK) return st;
+    return exchange_launch(B, lc, grid, (int)xp->ex.size(), xp->axes, launch_step, base, s);
   }
   return GD_OK;
 }
@@ -228,33 +292,29 @@ extern "C" gd_status gd_batch_dock(gd_job **jobs_in, int n_jobs, const gd_mc_par
     rep_steps += (double)g.R * mc[k].steps_total;
   }
   int step = 0;
-  std::vector<GdExch> ex;
+  ExchPoint xp;
+  std::vector<int> round;
   while (step < steps_max && st == GD_OK) {
-    // graph path: whole segment, every job active, exchange at its end
+    // graph path: whole segment, every job active, exchange point at its end
     if (uniform && seg > 1 && step + seg <= steps_max && !stats) {
       hipError_t e = hipMemcpyAsync(B.d_base, &step, sizeof(int), hipMemcpyHostToDevice, s);
       if (e != hipSuccess) {
         st = hip_fail(e, "hipMemcpyAsync base");
         break;
       }
+      // the segment's exchange tables (the round is (step + seg) / seg - 1 for
+      // every job), uploaded before each replay
+      round.assign(B.n_jobs, mc[0].steps_per_exchange > 0 ? (step + seg) / seg - 1 : -1);
+      xp.set(B, round);
       if (!gexec) {
         if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
           st = hip_fail(e, "hipStreamBeginCapture");
           break;
         }
         for (int k = 0; k < seg && st == GD_OK; ++k) {
-          std::vector<GdExch> exk;
-          if (k == seg - 1 && mc[0].steps_per_exchange > 0)
-            for (int q = 0; q < n_jobs; ++q)
-              if (jobs[q]->h.n_tmp > 1) exk.push_back({q, -1});
-          // exchange entries with round -1 take the round from the base step
           st = batch_step(B, lc, grid, k, B.d_base, nullptr, s, nullptr);
-          if (st == GD_OK && !exk.empty()) {
-            // round of this segment = (base + seg) / seg - 1, resolved at replay
-            // by re-uploading the table before the graph launch (below)
-            hipLaunchKernelGGL(gd_exchange_kernel, dim3((int)exk.size()), dim3(256), 0, s,
-                               B.d_jobs, B.d_ex);
-          }
+          if (st == GD_OK && k == seg - 1 && !xp.empty())
+            st = exchange_launch(B, lc, grid, (int)xp.ex.size(), xp.axes, k, B.d_base, s);
         }
         hipError_t e2 = hipStreamEndCapture(s, &graph);
         if (st == GD_OK && e2 != hipSuccess) st = hip_fail(e2, "hipStreamEndCapture");
@@ -264,23 +324,13 @@ extern "C" gd_status gd_batch_dock(gd_job **jobs_in, int n_jobs, const gd_mc_par
           break;
         }
       }
-      if (mc[0].steps_per_exchange > 0) {
-        ex.clear();
-        const int round = (step + seg) / seg - 1;
-        for (int q = 0; q < n_jobs; ++q)
-          if (jobs[q]->h.n_tmp > 1) ex.push_back({q, round});
-        if (!ex.empty() &&
-            (e = hipMemcpyAsync(B.d_ex, ex.data(), ex.size() * sizeof(GdExch), hipMemcpyHostToDevice, s)) != hipSuccess) {
-          st = hip_fail(e, "hipMemcpyAsync exch");
-          break;
-        }
-      }
+      if (!xp.empty() && (st = exchange_upload(B, xp, s)) != GD_OK) break;
       if ((e = hipGraphLaunch(gexec, s)) != hipSuccess) {
         st = hip_fail(e, "hipGraphLaunch");
         break;
       }
-      // the host-side copies into d_base / d_ex must not be overwritten while
-      // the graph may still read them: keep one segment in flight
+      // the host-side copies into d_base and the exchange tables must not be
+      // overwritten while the graph may still read them: keep one segment in flight
       if ((e = hipStreamSynchronize(s)) != hipSuccess) {
         st = hip_fail(e, "hipStreamSynchronize");
         break;
@@ -296,16 +346,16 @@ extern "C" gd_status gd_batch_dock(gd_job **jobs_in, int n_jobs, const gd_mc_par
       // shrink the lists to the jobs still running; their step0 stays 0
       if ((st = B.build(jobs, act, s)) != GD_OK) break;
     }
-    ex.clear();
+    round.assign(act.size(), -1);
     for (int q = 0; q < (int)act.size(); ++q) {
-      const gd_mc_param &m = mc[act[q]];
-      if (m.steps_per_exchange > 0 && jobs[act[q]]->h.n_tmp > 1 && (step + 1) % m.steps_per_exchange == 0)
-        ex.push_back({q, (step + 1) / m.steps_per_exchange - 1});
+      const int spe = mc[act[q]].steps_per_exchange;
+      if (spe > 0 && (step + 1) % spe == 0) round[q] = (step + 1) / spe - 1;
     }
-    st = batch_step(B, lc, grid, step, nullptr, &ex, s, stats ? &tp : nullptr);
+    xp.set(B, round);
+    st = batch_step(B, lc, grid, step, nullptr, &xp, s, stats ? &tp : nullptr);
     if (st != GD_OK) break;
-    if (!ex.empty() || (stats && (step + 1) % 64 == 0)) {
-      hipError_t e = hipStreamSynchronize(s);
+    if (!xp.empty() || (stats && (step + 1) % 64 == 0)) {
+      hipError_t e = hipStreamSynchronize(s);  // exchange tables are rewritten next time
       if (e != hipSuccess) {
         st = hip_fail(e, "hipStreamSynchronize");
         break;
@@ -615,15 +665,16 @@ extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
       if ((st = s->lists.build(jobs, act, hs)) != GD_OK) return st;
       s->dirty = false;
     }
-    std::vector<GdExch> ex;
+    std::vector<int> round(act.size(), -1);
     for (int q = 0; q < (int)act.size(); ++q) {
       ServJob *j = s->active[act[q]];
       const int spe = j->mc.steps_per_exchange;
-      if (spe > 0 && j->job->h.n_tmp > 1 && (j->done_steps + 1) % spe == 0)
-        ex.push_back({q, (j->done_steps + 1) / spe - 1});
+      if (spe > 0 && (j->done_steps + 1) % spe == 0) round[q] = (j->done_steps + 1) / spe - 1;
     }
-    st = batch_step(s->lists, s->lc, pair_grid(pair_waves_per_cu(s->lc)), s->launch_step, nullptr, &ex, hs, nullptr);
-    if (!ex.empty()) (void)hipStreamSynchronize(hs);  // ex table is reused next step
+    ExchPoint xp;
+    xp.set(s->lists, round);
+    st = batch_step(s->lists, s->lc, pair_grid(pair_waves_per_cu(s->lc)), s->launch_step, nullptr, &xp, hs, nullptr);
+    if (!xp.empty()) (void)hipStreamSynchronize(hs);  // exchange tables are reused next step
     ++s->launch_step;
     for (int q : act) s->active[q]->done_steps += 1;
     // jobs whose step counts end here leave the lists at the next step

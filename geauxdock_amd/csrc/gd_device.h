@@ -19,9 +19,9 @@
 #define GD_PTAB_W 12      // floats per (ligand type, point type) entry (9 used)
 #define GD_PTAB_PT 32     // point-type stride of the table (31 used)
 #define GD_REC_W 17       // floats per accept record: step, movematrix[6], energy[10]
-#define GD_NLIN 6
+#define GD_NLIN 6         // linear partial sums: vdw ele pmf psp hdb kde
 #define GD_GROUP 8        // protein tiles (waves) per pair-kernel work group, summed in LDS
-#define GD_RK 16          // reduction slices (work groups) per 64-replica block         // linear partial sums: vdw ele pmf psp hdb kde
+#define GD_RK 16          // reduction slices (work groups) per 64-replica block
 
 // point flag bits (pattr.z as int)
 #define GD_PF_TYPE 0x1f
@@ -73,6 +73,8 @@ struct GdJob {
   int *ligi, *prti, *tmpi;// [Rp] conformer / temperature indices
   int *acc;               // [Rp] last step accepted
   int *nacc, *nexc;       // [Rp] counters
+  int *nswp;              // [2][Rp] accepted pose swaps (ligand axis, protein axis)
+  float *swp_e;           // [Rp] total energy before a swap sub-step
   float *best;            // [Rp]
   int *own;               // [n_prt*n_lig][n_tmp] replica at each temperature
   float *coords;          // [L][3][Rp] transformed ligand of the current proposal
@@ -96,6 +98,7 @@ struct GdJob {
   int move_mode;
   uint32_t seed_lo, seed_hi, job_id;
   int record;
+  int xaxes;              // exchange axes (GD_EXCH_* bits)
   int step0;              // launch step at which this job's MC step 0 ran
   int steps_total;
   int *wq;                // work-queue counter of the persistent pair kernel

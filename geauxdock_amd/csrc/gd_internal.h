@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,14 @@ gd_status launch_finish(const GdJob *d_jobs, const GdBlock *d_blocks, int n_bloc
                         int step, hipStream_t s);
 gd_status launch_transform(const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks, hipStream_t s);
 gd_status launch_exchange(const GdJob *d_jobs, const GdExch *d_ex, int n_ex, hipStream_t s);
+// One conformer swap sub-step along `axis` (0 ligand, 1 protein) for the jobs
+// of the lists whose swr[job] round is >= 0: propose, pair (the caller's
+// launcher), reduce, finish(EVAL), accept.  launch_step / step_base as for the
+// finish kernel of the step just completed; wq is reset for the next pair
+// launch when given.
+gd_status launch_swap(int axis, const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks,
+                      const int *d_swr, int launch_step, const int *step_base, int *wq,
+                      const std::function<gd_status()> &pair, hipStream_t s);
 
 // HIP-event timer for one kernel over many launches: events are recorded
 // around every launch and summed at synchronisation points, so timing never

@@ -745,3 +745,146 @@ __global__ __launch_bounds__(256) void gd_exchange_kernel(const GdJob *__restric
     }
   }
 }
+
+// ---------------------------------------------------------------------------
+// Conformer pose swaps (Hamiltonian replica exchange along the ligand- or
+// protein-conformer axis).  In round k, along axis `axis`, the lines of
+// replicas that share the other conformer index and the temperature pair
+// conformers (c, c+1) for c = k mod 2, c+2, ...; the two replicas of a pair
+// propose each other's pose.  One swap sub-step is
+//   gd_swap_propose_kernel  (prop := partner's pose, transform)
+//   pair + reduce + finish(EVAL)  (energy of every replica at that pose)
+//   gd_swap_accept_kernel   (joint Metropolis decision, next step's proposal)
+// A job takes part when swr[job] >= 0 (the round) and its exchange axes hold
+// `axis`; otherwise both kernels leave it untouched, and the evaluation in
+// between changes nothing of its state.
+
+// Partner of replica r along `axis` in round `rnd`, or -1; *lower is set when
+// r holds the lower conformer index of the pair.
+__device__ __forceinline__ int gd_swap_partner(GdJobC &J, int r, int axis, int rnd, bool *lower) {
+  *lower = false;
+  if (r >= J.R) return -1;
+  const int l = gd_g(J.ligi)[r], p = gd_g(J.prti)[r], t = gd_g(J.tmpi)[r];
+  const int n = axis ? J.n_prt : J.n_lig, c = axis ? p : l;
+  const int c0 = c - (rnd & 1);
+  if (c0 < 0) return -1;
+  const int cq = (c0 & 1) ? c - 1 : c + 1;
+  if (cq >= n) return -1;
+  *lower = c < cq;
+  const int ladder = axis ? cq * J.n_lig + l : p * J.n_lig + cq;
+  return gd_g(J.own)[(size_t)ladder * J.n_tmp + t];
+}
+
+__device__ __forceinline__ bool gd_swap_active(GdJobC &J, int rnd, int axis) {
+  return rnd >= 0 && ((J.xaxes >> (1 + axis)) & 1);  // GD_EXCH_LIG = 2, GD_EXCH_PRT = 4
+}
+
+__global__ __launch_bounds__(256) void gd_swap_propose_kernel(const GdJob *__restrict__ jobs,
+                                                               const GdBlock *__restrict__ blocks,
+                                                               int axis, const int *__restrict__ swr) {
+  const GdBlock bk = gd_c(blocks)[blockIdx.x];
+  GdJobC &J = gd_c(jobs)[bk.job];
+  const int rnd = gd_c(swr)[bk.job];
+  if (!gd_swap_active(J, rnd, axis)) return;
+  const int lane = threadIdx.x & 63, wave = gd_uniform((int)(threadIdx.x >> 6));
+  const int r = bk.rb + lane;
+  const int Rp = J.Rp;
+  bool lower;
+  const int q = gd_swap_partner(J, r, axis, rnd, &lower);
+  // a paired lane takes its partner's pose; the others keep their proposal
+  // (prop of a paired lane is written below, never read here)
+  GD_AS_GLOBAL float *prop = gd_g(J.prop);
+  const GD_AS_GLOBAL float *mm = gd_g(J.mm);
+  float m[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) m[i] = q >= 0 ? mm[(size_t)i * Rp + q] : prop[(size_t)i * Rp + r];
+  if (wave == 0 && q >= 0) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) prop[(size_t)i * Rp + r] = m[i];
+    gd_g(J.swp_e)[r] = gd_g(J.en)[(size_t)9 * Rp + r];
+  }
+  float R[9];
+  gd_rotation(m, R);
+  const float t[3] = {m[0] + J.center[0], m[1] + J.center[1], m[2] + J.center[2]};
+  gd_transform_atoms(J, r, R, t, wave, 4);
+}
+
+// Joint decision of each pair: with E the totals before the swap and T the
+// totals at the partner's pose, both members accept iff
+//   u < exp(mb_a (T_a - E_a) + mb_b (T_b - E_b)),
+// a the lower-conformer member (both lanes evaluate the same expression), u
+// from gd_swap_u01.  Accepted members take the pose and its ten terms.  Every
+// lane then re-proposes the next MC step from its (possibly new) pose with the
+// step's own draws and transforms it.
+__global__ __launch_bounds__(256) void gd_swap_accept_kernel(const GdJob *__restrict__ jobs,
+                                                              const GdBlock *__restrict__ blocks,
+                                                              int axis, const int *__restrict__ swr,
+                                                              int launch_step,
+                                                              const int *__restrict__ step_base) {
+  __shared__ float xf[12][64];
+  const GdBlock bk = gd_c(blocks)[blockIdx.x];
+  GdJobC &J = gd_c(jobs)[bk.job];
+  const int rnd = gd_c(swr)[bk.job];
+  if (!gd_swap_active(J, rnd, axis)) return;  // uniform over the block
+  if (step_base) launch_step += *step_base;
+  const int lane = threadIdx.x & 63, wave = gd_uniform((int)(threadIdx.x >> 6));
+  const int r = bk.rb + lane;
+  const int Rp = J.Rp;
+  if (wave == 0) {
+    GD_AS_GLOBAL float *prop = gd_g(J.prop), *mm = gd_g(J.mm), *en = gd_g(J.en);
+    bool lower;
+    const int q = gd_swap_partner(J, r, axis, rnd, &lower);
+    float m[6];
+    if (q >= 0) {
+      const int a = lower ? r : q, b = lower ? q : r;
+      const GD_AS_GLOBAL float *ev = gd_g(J.ev_terms), *e0 = gd_g(J.swp_e), *mb = gd_g(J.minus_beta);
+      const GD_AS_GLOBAL int *tmpi = gd_g(J.tmpi);
+      // products rounded separately (no fused multiply-add): for identical
+      // conformers T_a - E_a = -(T_b - E_b) and x is exactly 0
+      const float x = __fmul_rn(mb[tmpi[a]], ev[(size_t)a * 10 + 9] - e0[a]) +
+                      __fmul_rn(mb[tmpi[b]], ev[(size_t)b * 10 + 9] - e0[b]);
+      const float u = gd_swap_u01(((uint64_t)J.seed_hi << 32) | J.seed_lo, J.job_id, (uint32_t)a,
+                                  (uint32_t)rnd, (uint32_t)axis);
+      if (u < expf(x)) {
+        float e[10];
+#pragma unroll
+        for (int i = 0; i < 10; ++i) e[i] = ev[(size_t)r * 10 + i];
+#pragma unroll
+        for (int i = 0; i < 10; ++i) en[(size_t)i * Rp + r] = e[i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) mm[(size_t)i * Rp + r] = prop[(size_t)i * Rp + r];
+        GD_AS_GLOBAL float *best = gd_g(J.best);
+        best[r] = fminf(best[r], e[9]);
+        gd_g(J.nswp)[(size_t)axis * Rp + r] += 1;
+        gd_g(J.acc)[r] = 1;
+      }
+      const int step = launch_step - J.step0;  // the MC step just finished
+      float inc[6], un;
+      gd_move(J, r, (uint32_t)step + 1u, inc, &un);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const float v = inc[i] + mm[(size_t)i * Rp + r];
+        prop[(size_t)i * Rp + r] = v;
+        m[i] = v;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) m[i] = prop[(size_t)i * Rp + r];
+    }
+    float R[9];
+    gd_rotation(m, R);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) xf[i][lane] = R[i];
+    xf[9][lane] = m[0] + J.center[0];
+    xf[10][lane] = m[1] + J.center[1];
+    xf[11][lane] = m[2] + J.center[2];
+  }
+  __syncthreads();
+  float R[9], t[3];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R[i] = xf[i][lane];
+  t[0] = xf[9][lane];
+  t[1] = xf[10][lane];
+  t[2] = xf[11][lane];
+  gd_transform_atoms(J, r, R, t, wave, 4);
+}

@@ -13,6 +13,8 @@
 //   counter = (step, replica, job_id, slot)
 //   slot 0  -> move draws 0..3, slot 1 -> move draws 4..5, accept draw, spare
 //   slot 2  -> temperature-exchange draw (counter word 0 = round, word 1 = pair)
+//   slot 3  -> ligand-conformer swap draw, slot 4 -> protein-conformer swap draw
+//              (counter word 0 = round, word 1 = lower-conformer replica)
 #pragma once
 #include <stdint.h>
 
@@ -80,5 +82,15 @@ GD_HD void gd_step_draws(uint64_t seed, uint32_t job, uint32_t replica, uint32_t
 // Exchange draw for pair `pair` in exchange round `round`.
 GD_HD float gd_exchange_u01(uint64_t seed, uint32_t job, uint32_t pair, uint32_t round) {
   const gd_u32x4 a = gd_philox4x32(round, pair, job, 2u, (uint32_t)seed, (uint32_t)(seed >> 32));
+  return gd_u01(a.v[0]);
+}
+
+// Draw of a conformer pose swap in exchange round `round` along `axis`
+// (0: ligand conformer, 1: protein conformer); the pair is named by its member
+// on the lower conformer index.
+GD_HD float gd_swap_u01(uint64_t seed, uint32_t job, uint32_t replica, uint32_t round,
+                        uint32_t axis) {
+  const gd_u32x4 a =
+      gd_philox4x32(round, replica, job, 3u + axis, (uint32_t)seed, (uint32_t)(seed >> 32));
   return gd_u01(a.v[0]);
 }

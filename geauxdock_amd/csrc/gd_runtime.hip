@@ -33,6 +33,9 @@ __global__ void gd_transform_kernel(const GdJob *, const GdBlock *, int *);
 __global__ void gd_reduce_kernel(const GdJob *, const GdBlock *);
 __global__ void gd_finish_kernel(const GdJob *, const GdBlock *, int, int, const int *, int *);
 __global__ void gd_exchange_kernel(const GdJob *, const GdExch *);
+__global__ void gd_swap_propose_kernel(const GdJob *, const GdBlock *, int, const int *);
+__global__ void gd_swap_accept_kernel(const GdJob *, const GdBlock *, int, const int *, int,
+                                      const int *);
 
 namespace gdr {
 
@@ -402,6 +405,8 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
   a.add(&g.acc, (size_t)Rp);
   a.add(&g.nacc, (size_t)Rp);
   a.add(&g.nexc, (size_t)Rp);
+  a.add(&g.nswp, (size_t)2 * Rp);
+  a.add(&g.swp_e, (size_t)Rp);
   a.add(&g.best, (size_t)Rp);
   a.add(&g.own, (size_t)nl * np * nt);
   a.add(&g.coords, (size_t)3 * L * Rp);
@@ -452,7 +457,8 @@ size_t footprint(const gd_job_desc *d) {
              a((nty + 1) * GD_PTAB_PT * GD_PTAB_W * 4) + a((nty + 1) * P * 4) +
              a(std::max<size_t>(1, K) * 16) + a(std::max<size_t>(1, M) * Ls * 16) +
              a(std::max<size_t>(1, M) * 8) + a(nt * 4);
-  b += a(6 * Rp * 4) * 2 + a(10 * Rp * 4) + a(Rp * 4) * 6 + a(Rp * 4) + a(nl * np * nt * 4) +
+  b += a(6 * Rp * 4) * 2 + a(10 * Rp * 4) + a(Rp * 4) * 6 + a(2 * Rp * 4) + a(Rp * 4) +
+       a(Rp * 4) + a(nl * np * nt * 4) +
        a(3 * L * Rp * 4) + a(Rp * 4) + a(10 * Rp * 4) + a(9 * Rp * 4) + a(64 * 4);
   int lc, ptile;
   default_tiling((int)L, (int)P, (int)R, lc, ptile);
@@ -487,6 +493,8 @@ gd_status reset_replicas(gd_job *j) {
   GD_UP(g.acc, zero.data(), Rp * sizeof(int));
   GD_UP(g.nacc, zero.data(), Rp * sizeof(int));
   GD_UP(g.nexc, zero.data(), Rp * sizeof(int));
+  GD_UP(g.nswp, zero.data(), Rp * sizeof(int));
+  GD_UP(g.nswp + Rp, zero.data(), Rp * sizeof(int));
   GD_UP(g.rec_n, zero.data(), Rp * sizeof(int));
   GD_UP(g.mm, zf.data(), (size_t)6 * Rp * sizeof(float));
   GD_UP(g.prop, zf.data(), (size_t)6 * Rp * sizeof(float));
@@ -568,6 +576,25 @@ gd_status launch_exchange(const GdJob *d_jobs, const GdExch *d_ex, int n_ex, hip
   return e == hipSuccess ? GD_OK : hip_fail(e, "exchange kernel launch");
 }
 
+gd_status launch_swap(int axis, const GdJob *d_jobs, const GdBlock *d_blocks, int n_blocks,
+                      const int *d_swr, int launch_step, const int *step_base, int *wq,
+                      const std::function<gd_status()> &pair, hipStream_t s) {
+  if (n_blocks <= 0) return GD_OK;
+  hipLaunchKernelGGL(gd_swap_propose_kernel, dim3(n_blocks), dim3(256), 0, s, d_jobs, d_blocks, axis,
+                     d_swr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "swap propose kernel launch");
+  gd_status st = pair();
+  if (st != GD_OK) return st;
+  hipLaunchKernelGGL(gd_reduce_kernel, dim3(n_blocks * GD_RK), dim3(256), 0, s, d_jobs, d_blocks);
+  hipLaunchKernelGGL(gd_finish_kernel, dim3(n_blocks), dim3(1024), 0, s, d_jobs, d_blocks, GD_MODE_EVAL,
+                     0, (const int *)nullptr, wq);
+  hipLaunchKernelGGL(gd_swap_accept_kernel, dim3(n_blocks), dim3(256), 0, s, d_jobs, d_blocks, axis,
+                     d_swr, launch_step, step_base);
+  e = hipGetLastError();
+  return e == hipSuccess ? GD_OK : hip_fail(e, "swap kernel launch");
+}
+
 void set_mc(GdJob &g, const gd_mc_param *mc) {
   for (int i = 0; i < 6; ++i) g.move_scale[i] = mc->move_scale[i];
   g.move_mode = mc->move_mode;
@@ -575,6 +602,7 @@ void set_mc(GdJob &g, const gd_mc_param *mc) {
   g.seed_hi = (uint32_t)(mc->seed >> 32);
   g.job_id = mc->job_id;
   g.record = mc->record;
+  g.xaxes = mc->exchange_axes ? mc->exchange_axes : GD_EXCH_TEMP;
   g.steps_total = mc->steps_total;
   g.step0 = 0;
 }
@@ -611,7 +639,7 @@ gd_status read_out(gd_job *j, gd_replica_out *out) {
     int *dst;
     int *src;
   } ints[] = {{out->idx_tmp, g.tmpi}, {out->n_accept, g.nacc}, {out->n_exchange, g.nexc},
-              {out->is_accept, g.acc}};
+              {out->is_accept, g.acc}, {out->n_swap_lig, g.nswp}, {out->n_swap_prt, g.nswp + Rp}};
   for (auto &q : ints)
     if (q.dst) {
       if ((st = down(iv.data(), q.src, (size_t)Rp * sizeof(int))) != GD_OK) return st;
@@ -866,21 +894,36 @@ gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stat
   hipError_t e = hipMemcpyAsync(j->d_job, &g, sizeof(GdJob), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync job");
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  // exchange points: after MC steps spe-1, 2spe-1, ...; round k uses table entry k
   const int spe = mc->steps_per_exchange;
-  const bool exch = spe > 0 && g.n_tmp > 1;
-  const int n_rounds = exch ? mc->steps_total / spe : 0;
+  const bool tx = spe > 0 && (g.xaxes & GD_EXCH_TEMP) && g.n_tmp > 1;
+  const bool lx = spe > 0 && (g.xaxes & GD_EXCH_LIG) && g.n_lig > 1;
+  const bool px = spe > 0 && (g.xaxes & GD_EXCH_PRT) && g.n_prt > 1;
+  const int n_rounds = (tx || lx || px) ? mc->steps_total / spe : 0;
+  void *d_xt = nullptr;  // GdExch [n_rounds], then the swap rounds int [n_rounds]
   GdExch *d_ex = nullptr;
+  int *d_swr = nullptr;
   if (n_rounds > 0) {
-    std::vector<GdExch> ex(n_rounds);
-    for (int k = 0; k < n_rounds; ++k) ex[k] = {0, k};
-    e = hipMalloc(&d_ex, sizeof(GdExch) * n_rounds);
+    const size_t be = ((size_t)n_rounds * sizeof(GdExch) + 255) / 256 * 256;
+    std::vector<char> xt(be + (size_t)n_rounds * sizeof(int));
+    for (int k = 0; k < n_rounds; ++k) {
+      const GdExch x = {0, k};
+      std::memcpy(xt.data() + k * sizeof(GdExch), &x, sizeof x);
+      std::memcpy(xt.data() + be + k * sizeof(int), &k, sizeof k);
+    }
+    e = hipMalloc(&d_xt, xt.size());
     if (e != hipSuccess) return hip_fail(e, "hipMalloc exch");
-    e = hipMemcpy(d_ex, ex.data(), sizeof(GdExch) * n_rounds, hipMemcpyHostToDevice);
+    e = hipMemcpy(d_xt, xt.data(), xt.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-      hipFree(d_ex);
+      hipFree(d_xt);
       return hip_fail(e, "hipMemcpy exch");
     }
+    d_ex = (GdExch *)d_xt;
+    d_swr = (int *)((char *)d_xt + be);
   }
+  const std::function<gd_status()> pair = [&]() {
+    return launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s);
+  };
   KTimer tp;
   const bool timing = stats != nullptr;
   const auto t0 = std::chrono::steady_clock::now();
@@ -902,8 +945,16 @@ gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stat
       if ((st = launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s)) != GD_OK) break;
       if (timing && (st = tp.end(s)) != GD_OK) break;
       if ((st = launch_finish(j->d_job, j->d_blocks, j->n_blocks, GD_MODE_STEP, step, s)) != GD_OK) break;
-      if (d_ex && (step + 1) % spe == 0 && (step + 1) / spe <= n_rounds)
-        if ((st = launch_exchange(j->d_job, d_ex + ((step + 1) / spe - 1), 1, s)) != GD_OK) break;
+      if (n_rounds > 0 && (step + 1) % spe == 0 && (step + 1) / spe <= n_rounds) {
+        const int k = (step + 1) / spe - 1;
+        if (tx && (st = launch_exchange(j->d_job, d_ex + k, 1, s)) != GD_OK) break;
+        for (int axis = 0; axis < 2; ++axis)
+          if ((axis ? px : lx) &&
+              (st = launch_swap(axis, j->d_job, j->d_blocks, j->n_blocks, d_swr + k, step, nullptr,
+                                nullptr, pair, s)) != GD_OK)
+            break;
+        if (st != GD_OK) break;
+      }
     }
     e = hipStreamSynchronize(s);  // segment boundary (the reference's blocking D2H)
     if (st == GD_OK && e != hipSuccess) st = hip_fail(e, "MC segment");
@@ -914,7 +965,7 @@ gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stat
     if (st == GD_OK && e2 != hipSuccess) st = hip_fail(e2, "hipStreamSynchronize");
   }
   const auto t1 = std::chrono::steady_clock::now();
-  if (d_ex) hipFree(d_ex);
+  if (d_xt) hipFree(d_xt);
   if (st != GD_OK) return st;
   if (stats) {
     stats->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();

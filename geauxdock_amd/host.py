@@ -143,7 +143,8 @@ def synth_complex(prt_npoint, lig_natom, n_lig, n_prt, n_tmp, floor_temp=0.004,
 
 
 def make_mc(steps_total, steps_per_dump=None, steps_per_exchange=0, move_mode="gaussian",
-            move_scale=None, seed=1, job_id=0, record=False):
+            move_scale=None, seed=1, job_id=0, record=False, axes=("temp",)):
+    """MC parameters; `axes` names the exchange axes ("temp", "lig", "prt")."""
     mc = abi.gd_mc_param()
     mc.steps_total = steps_total
     mc.steps_per_dump = steps_per_dump or max(1, steps_total)
@@ -155,6 +156,8 @@ def make_mc(steps_total, steps_per_dump=None, steps_per_exchange=0, move_mode="g
     mc.seed = seed
     mc.job_id = job_id
     mc.record = 1 if record else 0
+    bits = {"temp": abi.GD_EXCH_TEMP, "lig": abi.GD_EXCH_LIG, "prt": abi.GD_EXCH_PRT}
+    mc.exchange_axes = sum(bits[a] for a in set(axes))
     return mc
 
 
@@ -169,10 +172,13 @@ class ReplicaOut:
         self.n_exchange = np.zeros(n_rep, np.int32)
         self.best_energy = np.zeros(n_rep, np.float32)
         self.is_accept = np.zeros(n_rep, np.int32)
+        self.n_swap_lig = np.zeros(n_rep, np.int32)
+        self.n_swap_prt = np.zeros(n_rep, np.int32)
         o = abi.gd_replica_out()
         o.movematrix, o.energy = _f(self.movematrix), _f(self.energy)
         o.idx_tmp, o.n_accept, o.n_exchange = _i(self.idx_tmp), _i(self.n_accept), _i(self.n_exchange)
         o.best_energy, o.is_accept = _f(self.best_energy), _i(self.is_accept)
+        o.n_swap_lig, o.n_swap_prt = _i(self.n_swap_lig), _i(self.n_swap_prt)
         self.c = o
 
 

@@ -109,15 +109,27 @@ typedef struct {
 enum { GD_MOVE_REF_UNIFORM = 0,   /* reference: m += scale * U[0,1) (kernel_cpp_montecarlo_soa.C:248-249) */
        GD_MOVE_GAUSSIAN = 1 };    /* m += scale * N(0,1) */
 
+/* Replica-exchange axes (gd_mc_param.exchange_axes bits; 0 means GD_EXCH_TEMP).
+ * At every exchange point, in this order:
+ *  - TEMP: neighbour temperatures swap labels inside each (ligand conformer,
+ *    protein conformer) ladder;
+ *  - LIG:  replicas at the same temperature and protein conformer on neighbour
+ *    ligand conformers swap poses (Hamiltonian exchange: one extra energy
+ *    evaluation of every replica at its partner's pose);
+ *  - PRT:  the same along the protein-conformer axis.
+ * The reference declares exchange (geauxdock.h:340-369) but implements none. */
+enum { GD_EXCH_TEMP = 1, GD_EXCH_LIG = 2, GD_EXCH_PRT = 4 };
+
 typedef struct {
   int steps_total;          /* MC steps after the initial-energy pass */
   int steps_per_dump;       /* steps per record segment (reference STEPS_PER_DUMP) */
-  int steps_per_exchange;   /* temperature exchange period; 0 = off */
+  int steps_per_exchange;   /* exchange period in MC steps; 0 = off */
   int move_mode;            /* GD_MOVE_* */
   float move_scale[6];      /* tx ty tz rx ry rz */
   uint64_t seed;            /* Philox4x32-10 key */
   uint32_t job_id;          /* Philox counter word 2: the job's stream */
   int record;               /* 1 = keep per-dump accept records (drop-in Dock) */
+  int exchange_axes;        /* GD_EXCH_* bits; 0 = GD_EXCH_TEMP */
 } gd_mc_param;
 
 /* Per-replica results (any pointer may be NULL).  Replica order is the
@@ -129,7 +141,9 @@ typedef struct {
   int *n_accept;            /* [n_rep] accepted MC moves */
   int *n_exchange;          /* [n_rep] accepted temperature swaps */
   float *best_energy;       /* [n_rep] lowest accepted total energy */
-  int *is_accept;           /* [n_rep] last step accepted */
+  int *is_accept;           /* [n_rep] last step (or conformer swap) accepted */
+  int *n_swap_lig;          /* [n_rep] accepted pose swaps along the ligand-conformer axis */
+  int *n_swap_prt;          /* [n_rep] accepted pose swaps along the protein-conformer axis */
 } gd_replica_out;
 
 typedef struct {

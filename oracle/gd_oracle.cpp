@@ -20,7 +20,8 @@
 // (geauxdock.h:340-369), specified as: every steps_per_exchange steps, in round
 // k, pair temperatures (t, t+1) with t = k mod 2, t+2, ... inside every
 // (ligand conformer, protein conformer) ladder and swap their temperature
-// labels iff u < exp((mb[t+1] - mb[t]) * (E_i - E_j)).
+// labels iff u < exp((mb[t+1] - mb[t]) * (E_i - E_j)); then, per enabled
+// conformer axis, pose swaps between neighbour conformers (swap_axis below).
 //
 // Arithmetic modes:
 //   FP32 "gpu" rounding  : the squared distance is fmaf(dx,dx,fmaf(dy,dy,dz*dz))
@@ -352,9 +353,58 @@ void exchange_ladder(const gd_mc_param *mc, const float *mb, int nt, uint32_t la
   }
 }
 
+// Partner of replica r for a pose swap along `axis` (0: ligand conformer,
+// 1: protein conformer) in round `round`: the replicas that share the other
+// conformer index and the temperature form a line; conformers (c, c+1) pair
+// for c = round mod 2, c+2, ...  Returns -1 when r is unpaired.
+int swap_partner(int r, int axis, int round, int nl, int np, int nt, const int *lig, const int *prt,
+                 const int *tmp, const int *owner, bool *lower) {
+  const int l = lig[r], p = prt[r], t = tmp[r];
+  const int n = axis ? np : nl, c = axis ? p : l;
+  *lower = false;
+  if (c - (round & 1) < 0) return -1;
+  const int cq = ((c - (round & 1)) & 1) ? c - 1 : c + 1;
+  if (cq >= n) return -1;
+  *lower = c < cq;
+  const int p2 = axis ? cq : p, l2 = axis ? l : cq;
+  return owner[((size_t)p2 * nl + l2) * nt + t];
+}
+
+// Joint Metropolis decision of a pose-swap pair (a: lower-conformer member,
+// b: partner; E the totals before, T the totals at the partner's pose):
+// accept iff u < exp(mb_a (T_a - E_a) + mb_b (T_b - E_b)), u from Philox
+// counter (round, a, job, 3 + axis).
+bool swap_accept(const gd_mc_param *mc, int axis, int round, int a, float mba, float ta, float ea,
+                 float mbb, float tb, float eb) {
+  const P4 w = philox((uint32_t)round, (uint32_t)a, mc->job_id, 3u + (uint32_t)axis,
+                      (uint32_t)mc->seed, (uint32_t)(mc->seed >> 32), 10);
+  return u01(w.v[0]) < expf(mba * (ta - ea) + mbb * (tb - eb));
+}
+
 }  // namespace
 
 extern "C" {
+
+// Pose-swap rule on a frozen two-conformer toy: conformer 0 starts at pose X,
+// conformer 1 at pose Y; e[c][x] is conformer c's total at pose x (x: 0 = X,
+// 1 = Y).  Runs `rounds` attempts of the pair (replica 0 on conformer 0,
+// replica 1 on conformer 1, inverse temperature -mb for both) and returns the
+// swaps per attempt.
+double or_swap_toy(const float e[4], float mb, int rounds, uint64_t seed) {
+  gd_mc_param mc;
+  memset(&mc, 0, sizeof mc);
+  mc.seed = seed;
+  int x0 = 0;  // pose held by conformer 0; conformer 1 holds 1 - x0
+  long swaps = 0;
+  for (int k = 0; k < rounds; ++k) {
+    const int x1 = 1 - x0;
+    if (swap_accept(&mc, 0, k, 0, mb, e[0 * 2 + x1], e[0 * 2 + x0], mb, e[1 * 2 + x0], e[1 * 2 + x1])) {
+      x0 = x1;
+      ++swaps;
+    }
+  }
+  return rounds ? (double)swaps / rounds : 0.0;
+}
 
 // Exchange rule on a frozen ladder with the given total energies: runs
 // `rounds` exchange rounds, returns swaps per attempted pair and the final
@@ -459,8 +509,9 @@ int or_energy(const gd_job_desc *d, int n, const int *conf, const float *pose,
 }
 
 // Monte Carlo docking of one complex (FP32 shared rounding): initial-energy
-// pass with forced accept, steps_total steps, temperature exchange every
-// steps_per_exchange steps.  Replicas in SetReplica order (util.C:370-406).
+// pass with forced accept, steps_total steps, an exchange point every
+// steps_per_exchange steps (temperature exchange and conformer pose swaps as
+// mc->exchange_axes selects).  Replicas in SetReplica order (util.C:370-406).
 // rep_lo/rep_hi restrict the run to a replica range (for bounded timing
 // samples; exchange then needs the whole range of each ladder).  Optional
 // trace[n_rep][steps_total] receives the accepted total energy after each step.
@@ -474,7 +525,8 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
   if (rep_hi <= 0 || rep_hi > R) rep_hi = R;
   if (rep_lo < 0) rep_lo = 0;
   std::vector<float> mm((size_t)R * 6, 0.0f), en((size_t)R * 10, 0.0f), best(R, 1e30f);
-  std::vector<int> tmp(R), lig(R), prt(R), acc(R, 0), nacc(R, 0), nexc(R, 0);
+  std::vector<int> tmp(R), lig(R), prt(R), acc(R, 0), nacc(R, 0), nexc(R, 0), nswp((size_t)2 * R, 0);
+  const int axes = mc->exchange_axes ? mc->exchange_axes : GD_EXCH_TEMP;
   std::vector<int> owner((size_t)nl * np * nt);
   for (int p = 0; p < np; ++p)
     for (int t = 0; t < nt; ++t)
@@ -522,9 +574,10 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
         if (trace && !init) trace[(size_t)r * mc->steps_total + s] = en[(size_t)r * 10 + 9];
       }
     }
-    // temperature exchange after each full segment
-    if (!init && mc->steps_per_exchange > 0 && nt > 1 && s_end - s_begin == seg) {
-      const int round = s_end / seg - 1;
+    // exchange point after each full segment: temperatures, then conformer swaps
+    if (init || mc->steps_per_exchange <= 0 || s_end - s_begin != seg) continue;
+    const int round = s_end / seg - 1;
+    if ((axes & GD_EXCH_TEMP) && nt > 1) {
       for (int p = 0; p < np; ++p)
         for (int l = 0; l < nl; ++l) {
           int *own = &owner[((size_t)p * nl + l) * nt];
@@ -534,6 +587,47 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
           exchange_ladder(mc, d->minus_beta, nt, (uint32_t)((size_t)p * nl + l), round, own,
                           en.data(), tmp.data(), nexc.data());
         }
+    }
+    for (int axis = 0; axis < 2; ++axis) {
+      if (!(axes & (GD_EXCH_LIG << axis)) || (axis ? np : nl) < 2) continue;
+      // every member of a pair evaluates its own conformers at the partner's pose
+      std::vector<int> part(R, -1);
+      std::vector<char> low(R, 0);
+      for (int r = rep_lo; r < rep_hi; ++r) {
+        bool lower;
+        const int q = swap_partner(r, axis, round, nl, np, nt, lig.data(), prt.data(), tmp.data(),
+                                   owner.data(), &lower);
+        if (q >= rep_lo && q < rep_hi) part[r] = q, low[r] = lower;
+      }
+      std::vector<float> tr((size_t)R * 10, 0.0f);
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int r = rep_lo; r < rep_hi; ++r) {
+        if (part[r] < 0) continue;
+        std::vector<float> x(L), y(L), z(L);
+        Out<float> o;
+        const float *m = &mm[(size_t)part[r] * 6];
+        transform<float>(d, lig[r], m, x.data(), y.data(), z.data());
+        energy<float>(J, prt[r], m, x.data(), y.data(), z.data(), false, o);
+        for (int i = 0; i < 10; ++i) tr[(size_t)r * 10 + i] = o.term[i];
+      }
+      // joint decisions from the pre-swap state, then the updates
+      std::vector<char> take(R, 0);
+      for (int r = rep_lo; r < rep_hi; ++r) {
+        if (part[r] < 0) continue;
+        const int a = low[r] ? r : part[r], b = low[r] ? part[r] : r;
+        take[r] = swap_accept(mc, axis, round, a, d->minus_beta[tmp[a]], tr[(size_t)a * 10 + 9],
+                              en[(size_t)a * 10 + 9], d->minus_beta[tmp[b]], tr[(size_t)b * 10 + 9],
+                              en[(size_t)b * 10 + 9]);
+      }
+      const std::vector<float> mm0 = mm;
+      for (int r = rep_lo; r < rep_hi; ++r) {
+        if (!take[r]) continue;
+        for (int i = 0; i < 10; ++i) en[(size_t)r * 10 + i] = tr[(size_t)r * 10 + i];
+        for (int i = 0; i < 6; ++i) mm[(size_t)r * 6 + i] = mm0[(size_t)part[r] * 6 + i];
+        best[r] = std::min(best[r], tr[(size_t)r * 10 + 9]);
+        ++nswp[(size_t)axis * R + r];
+        acc[r] = 1;
+      }
     }
   }
   const auto t1 = std::chrono::steady_clock::now();
@@ -549,6 +643,8 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
       if (out->n_exchange) out->n_exchange[r] = nexc[r];
       if (out->best_energy) out->best_energy[r] = best[r];
       if (out->is_accept) out->is_accept[r] = acc[r];
+      if (out->n_swap_lig) out->n_swap_lig[r] = nswp[r];
+      if (out->n_swap_prt) out->n_swap_prt[r] = nswp[(size_t)R + r];
     }
   }
   return GD_OK;

@@ -1,6 +1,7 @@
 """GPU Monte Carlo path against the CPU oracle: initial pass, trajectories,
-acceptance statistics, temperature exchange, records, batching (persistent
-work queue + graph-captured segments) and the serving loop.
+acceptance statistics, temperature exchange, conformer pose swaps, records,
+batching (persistent work queue + graph-captured segments) and the serving
+loop.
 
 Trajectory parity: both sides draw the same Philox numbers, so the accept
 decisions agree step for step until a rounding-level energy difference lands
@@ -85,6 +86,57 @@ def test_exchange(ene, oracle):
     assert abs(a - b) <= 6 * math.sqrt(max(a, b)) + 4, (a, b)
 
 
+def test_swap_identical_conformers(ene):
+    """Identical conformers: a pose swap changes no energy, so every attempt is
+    accepted and the swap counts equal the pairing rule's attempt counts
+    exactly (pairing is bit-exact); temperatures stay put; each replica's
+    stored terms are, bit for bit, those gd_energy_eval gives for its final
+    pose."""
+    from tests.test_oracle import same_conformers, swap_attempts
+    nl, np_, nt = 4, 3, 2
+    job = g.synth_complex(600, 24, nl, np_, nt, floor_temp=0.05, ceiling_temp=0.2, seed=4, ene=ene)
+    same_conformers(job)
+    dev = g.DeviceJob(job)
+    mc = g.make_mc(40, steps_per_exchange=5, move_mode="gaussian", seed=3, axes=("lig", "prt"))
+    out, _ = dev.dock(mc)
+    r = np.arange(job.n_rep)
+    l, p = r % nl, r // (nl * nt)
+    assert np.array_equal(out.n_swap_lig, swap_attempts(nl, 8)[l])
+    assert np.array_equal(out.n_swap_prt, swap_attempts(np_, 8)[p])
+    assert np.array_equal(out.idx_tmp, (r // nl) % nt) and out.n_exchange.sum() == 0
+    terms, _, _ = dev.eval(np.stack([l, p], 1).astype(np.int32), out.movematrix)
+    assert np.array_equal(terms, out.energy)
+
+
+def test_swap_matches_oracle(ene, oracle):
+    """All three exchange axes: the GPU chain and the oracle's draw the same
+    Philox numbers, so moves, temperature swaps and pose swaps agree replica
+    for replica until a rounding-level energy difference meets a threshold."""
+    job = g.synth_complex(400, 16, 4, 3, 3, floor_temp=0.05, ceiling_temp=0.3, seed=8, ene=ene)
+    mc = g.make_mc(30, steps_per_exchange=5, move_mode="gaussian", seed=17,
+                   axes=("temp", "lig", "prt"))
+    out, _ = g.DeviceJob(job).dock(mc)
+    ref, _, _ = oracle.dock(job, mc, threads=8)
+    assert out.n_swap_lig.sum() > 0 and out.n_swap_prt.sum() > 0 and out.n_exchange.sum() > 0
+    same = ((out.n_accept == ref.n_accept) & (out.n_swap_lig == ref.n_swap_lig) &
+            (out.n_swap_prt == ref.n_swap_prt) & (out.idx_tmp == ref.idx_tmp) &
+            (np.abs(out.movematrix - ref.movematrix).max(1) < 1e-3))
+    assert same.mean() >= 0.8, same.mean()
+    e = np.abs(out.energy[same] - ref.energy[same]).max()
+    assert e < 1e-3, e
+
+
+def test_swap_statistics_match_oracle(ene, oracle):
+    job = g.synth_complex(400, 16, 4, 4, 2, floor_temp=0.05, ceiling_temp=0.3, seed=9, ene=ene)
+    mc = g.make_mc(300, steps_per_exchange=10, move_mode="gaussian", seed=23, axes=("lig", "prt"))
+    out, _ = g.DeviceJob(job).dock(mc)
+    ref, _, _ = oracle.dock(job, mc, threads=8)
+    for f in ("n_swap_lig", "n_swap_prt"):
+        a, b = getattr(out, f).sum(), getattr(ref, f).sum()
+        assert a % 2 == 0 and b % 2 == 0
+        assert abs(a - b) <= 6 * math.sqrt(max(a, b)) + 4, (f, a, b)
+
+
 def test_records(small_job):
     dev = g.DeviceJob(small_job)
     mc = g.make_mc(60, steps_per_dump=20, move_mode="reference", seed=3, record=True)
@@ -127,7 +179,8 @@ def test_batch_matches_plain_bitwise(ene):
     (persistent work-queue pair kernel, graph-replayed segments)."""
     hjobs = [g.synth_complex(P, L, 2, 3, 2, seed=s, ene=ene)
              for P, L, s in [(500, 12, 1), (900, 30, 2), (1600, 20, 3)]]
-    mcs = [g.make_mc(120, steps_per_exchange=20, move_mode="gaussian", seed=5, job_id=k)
+    axes = [("temp", "lig", "prt"), ("temp",), ("lig",)]
+    mcs = [g.make_mc(120, steps_per_exchange=20, move_mode="gaussian", seed=5, job_id=k, axes=axes[k])
            for k in range(3)]
     plain = []
     for h, mc in zip(hjobs, mcs):
@@ -143,12 +196,15 @@ def test_batch_matches_plain_bitwise(ene):
             assert np.array_equal(a.energy, b.energy)
             assert np.array_equal(a.movematrix, b.movematrix)
             assert np.array_equal(a.idx_tmp, b.idx_tmp)
+            assert np.array_equal(a.n_swap_lig, b.n_swap_lig)
+            assert np.array_equal(a.n_swap_prt, b.n_swap_prt)
         devs[0].eval(np.zeros((1, 2), np.int32), np.zeros((1, 6), np.float32))
 
 
 def test_batch_timed_path_matches_graph_path(ene):
     hjobs = [g.synth_complex(700, 16, 2, 2, 1, seed=s, ene=ene) for s in (4, 5)]
-    mcs = [g.make_mc(100, move_mode="gaussian", seed=9, job_id=k) for k in range(2)]
+    mcs = [g.make_mc(100, steps_per_exchange=20, move_mode="gaussian", seed=9, job_id=k,
+                     axes=("lig", "prt")) for k in range(2)]
     devs = [g.DeviceJob(h) for h in hjobs]
     a = _batch(devs, mcs, stats=False)
     b = _batch(devs, mcs, stats=True)
@@ -163,7 +219,8 @@ def test_server_alone_vs_mix(ene):
     target = g.synth_complex(800, 20, 3, 2, 2, seed=31, ene=ene)
     others = [g.synth_complex(P, L, 2, 2, 2, seed=s, ene=ene)
               for P, L, s in [(1500, 40, 32), (600, 12, 33), (1200, 60, 34)]]
-    mc_t = g.make_mc(150, steps_per_exchange=25, move_mode="gaussian", seed=77, job_id=1000)
+    mc_t = g.make_mc(150, steps_per_exchange=25, move_mode="gaussian", seed=77, job_id=1000,
+                     axes=("temp", "lig", "prt"))
 
     def serve(jobs_and_mc, sweep):
         cfg = abi.gd_server_config(hbm_budget=64 << 30, max_active=8, sweep_steps=sweep)
@@ -203,6 +260,8 @@ def test_server_alone_vs_mix(ene):
     assert np.array_equal(alone.energy, mixed.energy)
     assert np.array_equal(alone.movematrix, mixed.movematrix)
     assert np.array_equal(alone.idx_tmp, mixed.idx_tmp)
+    assert np.array_equal(alone.n_swap_lig, mixed.n_swap_lig)
+    assert alone.n_swap_lig.sum() > 0 and alone.n_swap_prt.sum() > 0
 
 
 def test_server_admission_budget(ene):

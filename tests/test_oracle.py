@@ -320,6 +320,84 @@ def test_exchange_ladder_is_permutation(oracle):
     assert tmp[nt - 1] == 0
 
 
+def test_swap_detailed_balance(oracle):
+    """Pose swap between two conformers with frozen energies e[c][x] (conformer
+    c at pose x; conformer 0 starts at X, 1 at Y).  Detailed balance of
+    u < exp(mb (T_a - E_a) + mb (T_b - E_b)) puts the swapped state at relative
+    weight p = exp(mb (e[0][Y] - e[0][X] + e[1][X] - e[1][Y])), so the swap rate
+    per attempt is 2 min(1, p) / (1 + p).  A wrong sign, a dropped partner term
+    or a decision taken from one member only all change the rate."""
+    L = oracle.lib
+    L.or_swap_toy.restype = C.c_double
+    L.or_swap_toy.argtypes = [C.POINTER(C.c_float), C.c_float, C.c_int, C.c_uint64]
+    mb = -1 / 0.4
+    for e in [(0.1, 0.5, 0.3, 0.2), (0.5, 0.1, 0.2, 0.6), (0.2, 0.2, 0.7, 0.7), (0.0, 0.9, 0.9, 0.0)]:
+        E = (C.c_float * 4)(*e)
+        n = 40000
+        rate = L.or_swap_toy(E, mb, n, 4321)
+        p = math.exp(mb * (e[1] - e[0] + e[2] - e[3]))
+        expect = 2 * min(1.0, p) / (1 + p)
+        assert abs(rate - expect) < 5 * math.sqrt(max(expect * (1 - expect), 1e-9) / n) + 1e-9, (e, rate)
+
+
+def swap_attempts(n, rounds):
+    """Attempts of each conformer index in `rounds` swap rounds: (c, c+1) for
+    c = k mod 2, c+2, ..."""
+    cnt = np.zeros(n, np.int64)
+    for k in range(rounds):
+        for c in range(k & 1, n - 1, 2):
+            cnt[c] += 1
+            cnt[c + 1] += 1
+    return cnt
+
+
+def same_conformers(job, lig=True, prt=True):
+    """Overwrite every ligand (protein) conformer with conformer 0, in place."""
+    d = job.desc
+    L, P = d.lig_natom, d.prt_npoint
+    for on, n, m, arrs in ((lig, d.n_lig, L, (d.lig_x, d.lig_y, d.lig_z)),
+                           (prt, d.n_prt, P, (d.prt_x, d.prt_y, d.prt_z))):
+        if not on:
+            continue
+        for a in arrs:
+            v = np.ctypeslib.as_array(a, (n * m,))
+            v[m:] = np.tile(v[:m], n - 1)
+
+
+def test_swap_identical_conformers(ene, oracle):
+    """With identical conformers a pose swap changes no energy (T_a = E_b and
+    T_b = E_a exactly), so the exponent is 0 and every attempt is accepted: the
+    swap counts equal the attempt counts of the pairing rule, temperatures stay
+    put, and every replica's stored terms are those of its final pose."""
+    job = g.synth_complex(300, 16, 4, 3, 2, floor_temp=0.05, ceiling_temp=0.2, seed=4, ene=ene)
+    same_conformers(job)
+    nl, np_, nt = 4, 3, 2
+    mc = g.make_mc(40, steps_per_exchange=5, move_mode="gaussian", seed=3, axes=("lig", "prt"))
+    out, _, _ = oracle.dock(job, mc, threads=8)
+    r = np.arange(job.n_rep)
+    l, p = r % nl, r // (nl * nt)
+    assert np.array_equal(out.n_swap_lig, swap_attempts(nl, 8)[l])
+    assert np.array_equal(out.n_swap_prt, swap_attempts(np_, 8)[p])
+    assert np.array_equal(out.idx_tmp, (r // nl) % nt) and out.n_exchange.sum() == 0
+    conf = np.stack([l, p], 1).astype(np.int32)
+    terms, _, _, _ = oracle.energy(job, conf, out.movematrix, mode=0)
+    assert np.array_equal(terms.astype(np.float32), out.energy)
+
+
+def test_swap_off_leaves_chain(ene, oracle):
+    """Temperature-only exchange (the default, axes 0) and an explicit
+    temperature axis give the same chain; conformer axes change it."""
+    job = g.synth_complex(300, 16, 3, 2, 3, floor_temp=0.05, ceiling_temp=0.2, seed=6, ene=ene)
+    a = g.make_mc(30, steps_per_exchange=5, seed=2)
+    a.exchange_axes = 0
+    b = g.make_mc(30, steps_per_exchange=5, seed=2, axes=("temp",))
+    c = g.make_mc(30, steps_per_exchange=5, seed=2, axes=("temp", "lig", "prt"))
+    oa, ob, oc = (oracle.dock(job, m, threads=8)[0] for m in (a, b, c))
+    assert np.array_equal(oa.energy, ob.energy) and np.array_equal(oa.idx_tmp, ob.idx_tmp)
+    assert oa.n_swap_lig.sum() == 0 and oc.n_swap_lig.sum() > 0 and oc.n_swap_prt.sum() > 0
+    assert oc.n_swap_lig.sum() % 2 == 0 and oc.n_swap_prt.sum() % 2 == 0
+
+
 def test_cpu_dock_entry_point(golden_job, oracle):
     """gd_cpu_dock (the CPU backend's Dock in C-ABI form): the initial pass of the
     golden complex reproduces the published line; a short run equals or_dock."""
