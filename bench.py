@@ -4,9 +4,11 @@
 Workload (the plain one-job-per-launch configuration): one synthetic complex,
 protein 3,000 effective points x 11 conformers, ligand 40 atoms x 20
 conformers, 8 temperatures on a geometric ladder 0.004..0.04 -> 1,760
-replicas; Gaussian moves N(0, 0.2 A) / N(0, 3 deg); temperature exchange every
-100 MC steps.  One bench "step" = one exchange interval (100 MC steps of all
-1,760 replicas + one exchange round), so the default --steps 100 is the whole
+replicas; Gaussian moves N(0, 0.2 A) / N(0, 3 deg); replica exchange every
+100 MC steps along all three axes (temperature labels, then pose swaps between
+neighbour ligand conformers and neighbour protein conformers, one extra energy
+pass per conformer axis).  One bench "step" = one exchange interval (100 MC
+steps of all 1,760 replicas + one exchange point), so the default --steps 100 is the whole
 10,000-step job.  FP32 arithmetic, synthetic data (seeded generator).
 
 Prints one JSON line (value = replica-steps/s).  The roofline object covers
@@ -31,6 +33,7 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 vector peak
 PRT_FLOPS, KDE_FLOPS, MCS_FLOPS = 62, 11, 9   # algorithmic flops per pair / match / entry
 P, L, NLIG, NPRT, NTMP = 3000, 40, 20, 11, 8
 STEPS_PER_EXCHANGE = 100
+AXES = ("temp", "lig", "prt")   # replica exchange across temperature x ligand x protein conformer
 SEED = 2
 
 
@@ -94,8 +97,8 @@ def batched(kind, n_jobs, mc_steps, plain_jobs):
     devs = [g.DeviceJob(h) for h in hosts]
     for d in devs:                       # shared-launch tiling: 4 atoms x 64 points
         d.set_tiling(4, min(64, d.host.desc.prt_npoint))
-    mcs = [g.make_mc(mc_steps, steps_per_exchange=100, move_mode="gaussian", seed=5, job_id=i)
-           for i in range(n_jobs)]
+    mcs = [g.make_mc(mc_steps, steps_per_exchange=100, move_mode="gaussian", seed=5, job_id=i,
+                     axes=AXES) for i in range(n_jobs)]
     arr = (C.c_void_p * n_jobs)(*[d.h.value for d in devs])
     mca = (abi.gd_mc_param * n_jobs)(*mcs)
     t0 = time.perf_counter()
@@ -127,8 +130,8 @@ def serve(rates, n_jobs, step_scale):
     steps = [int(rng.integers(2000, 10001) * step_scale) for _ in range(n_jobs)]
     hosts = [g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.004,
                              ceiling_temp=0.04, seed=500 + i) for i, z in enumerate(sizes)]
-    mcs = [g.make_mc(steps[i], steps_per_exchange=100, move_mode="gaussian", seed=7, job_id=i)
-           for i in range(n_jobs)]
+    mcs = [g.make_mc(steps[i], steps_per_exchange=100, move_mode="gaussian", seed=7, job_id=i,
+                     axes=AXES) for i in range(n_jobs)]
     for lam in rates:
         arr_t = np.cumsum(np.random.default_rng(int(lam)).exponential(1.0 / lam, n_jobs))
         cfg = abi.gd_server_config(hbm_budget=200 << 30, max_active=256, sweep_steps=50)
@@ -274,10 +277,10 @@ def main():
     # warmup: the same job for W exchange intervals (untimed)
     if args.warmup > 0:
         dev.dock(g.make_mc(args.warmup * STEPS_PER_EXCHANGE, steps_per_exchange=STEPS_PER_EXCHANGE,
-                           move_mode="gaussian", seed=SEED), stats=False)
+                           move_mode="gaussian", seed=SEED, axes=AXES), stats=False)
     steps = args.steps * STEPS_PER_EXCHANGE
     mc = g.make_mc(steps, steps_per_dump=steps, steps_per_exchange=STEPS_PER_EXCHANGE,
-                   move_mode="gaussian", seed=SEED)
+                   move_mode="gaussian", seed=SEED, axes=AXES)
     t0 = time.perf_counter()
     out, st = dev.dock(mc, stats=True)     # returns after the device is idle
     wall = time.perf_counter() - t0
@@ -305,10 +308,14 @@ def main():
         "config": {
             "workload": "plain one-job-per-launch: P=3000 points x 11 conformers, L=40 atoms x "
                         "20 conformers, 8 temperatures 0.004-0.04 (1760 replicas), 10000 MC steps, "
-                        "exchange every 100",
+                        "exchange every 100 (temperature, ligand conformer, protein conformer)",
             "replicas": job.n_rep, "mc_steps": steps, "prt_npoint": P, "lig_natom": L,
             "kde_npoint": job.desc.kde_npoint, "mcs_nrow": job.desc.mcs_nrow,
-            "step_unit": "100 MC steps + 1 exchange round",
+            "step_unit": "100 MC steps + 1 exchange point",
+            "exchange_axes": list(AXES),
+            "swaps_accepted": {"temperature": int(out.n_exchange.sum()) // 2,
+                               "ligand_conformer": int(out.n_swap_lig.sum()) // 2,
+                               "protein_conformer": int(out.n_swap_prt.sum()) // 2},
             "tiling": {"atoms_per_lane": dev.tiling()[0], "tile_points": dev.tiling()[1]},
             "device": name.value.decode(), "cu": ncu.value,
         },

@@ -58,7 +58,7 @@ struct GdJob {
   const float4 *pcoord;   // [n_prt][P]
   const float4 *pattr;    // [P] (ele, hpp, flags-as-int, 0)
   const float *ptab;      // [n_types + 1][GD_PTAB_PT][GD_PTAB_W] (last block: padding atoms):
-                          // p1a, -p2a, PMF exponent B and C,
+                          // 1/p1a, -p2a/p1a, negated PMF exponent B and C,
                           // pmf1, psp r^2 threshold (sign bit: H-bond active), H-bond c, -hdb0 c,
                           // far-field r^2 threshold
   const float *pspv;      // [n_types][P] psp value of class-2 points
@@ -92,6 +92,7 @@ struct GdJob {
   float kde2l;            // kde2 * log2(e)
   float kde3_inv;
   float inv_L;
+  float pmf_all;          // pmf1 summed over every (atom, point) pair of the job
   float center[3], pocket[3];
   float a_para[10], b_para[10], w[10];
   float move_scale[6];

@@ -109,7 +109,7 @@ __device__ __forceinline__ void gd_pair_atoms(
     asm volatile("" ::"s"(rw[0]), "s"(rw[1]), "s"(rw[2]), "s"(rw[3]), "s"(rw[4]), "s"(rw[5]),
                  "s"(rw[6]), "s"(rw[7]), "s"(rw[8])
                  : "memory");
-    const float p1a = rw[0], np2a = rw[1], pB = rw[2], pC = rw[3];
+    const float w9 = rw[0], nq = rw[1], pB = rw[2], pC = rw[3];
     const float pmf1 = rw[4], pthr = rw[5], hdb1c = rw[6], nhdb0c = rw[7], r2far = rw[8];
     if (j + 1 < LC) {
       const GD_AS_CONST float *__restrict__ prm =
@@ -140,32 +140,40 @@ __device__ __forceinline__ void gd_pair_atoms(
       }
     }
     // soft Lennard-Jones (:61-64): (p1 - p2) / (p1 lj0 (1 + lj1 r^2) + 1) with
-    // p1 = p1a / r^9, p2 = p2a / r^6, multiplied through by r^9:
-    // (p1a - p2a r^3) / (p1a lj0 (1 + lj1 r^2) + r^9)
+    // p1 = p1a / r^9, p2 = p2a / r^6, multiplied through by r^9 / p1a:
+    // (1 - q r^3) / (lj0 (1 + lj1 r^2) + w r^9), q = p2a / p1a, w = 1 / p1a
+    // (rows with p1a = p2a = 0 carry w = inf: the term is 0)
     {
       const float r3 = r2 * rr;
       const float r9 = (r3 * r3) * r3;
-      const float num = __builtin_fmaf(np2a, r3, p1a);
-      const float den = __builtin_fmaf(p1a, __builtin_fmaf(k.lj01, r2, k.lj0), r9);
+      const float num = __builtin_fmaf(nq, r3, 1.0f);
+      const float den = __builtin_fmaf(w9, r9, __builtin_fmaf(k.lj01, r2, k.lj0));
       evdw = __builtin_fmaf(num, gd_rcp(den), evdw);
+      // pinned here: sunk past the branches below, the terms of every atom of
+      // the iteration stayed live to its end (registers)
+      asm volatile("" : "+v"(evdw));
     }
-    // Far field, decided for the whole wave on one compare: beyond the row's
-    // r^2 threshold the PMF exponent is below -26, so 1 + exp2 of it rounds to 1
-    // and the PMF term is exactly pmf1, and s = el1 r >= 1, so the electrostatic
-    // factor is 1/s (scaled: 1/r)
+    // Contact PMF (:88-91): pmf1 / (1 + exp2(x)), x = (6 - r/2)(r - pmf0) log2(e).
+    // It is summed as pmf1 - pmf1 / (1 + exp2(-x)): the pmf1 of every pair of
+    // the job is one constant added once per replica, and epmf collects only the
+    // corrections.  Beyond the row's r^2 threshold x <= -26, so 1 + exp2(x)
+    // rounds to 1: the reference term is exactly pmf1 and the correction is
+    // taken as exactly 0, on every lane, so a replica's sum does not depend on
+    // the other lanes of its wave.  Also s = el1 r >= 1 there, so the
+    // electrostatic factor is 1/s (scaled: 1/r).  Far field, decided for the
+    // whole wave on one compare: no correction, g = 1/r.
     float g = ri;
-    if (__ballot(r2 < r2far) != 0) {
+    const bool near = r2 < r2far;
+    if (__ballot(near) != 0) {
       asm volatile("");
-      // contact PMF (:88-91): exponent (6 - r/2)(r - pmf0) log2(e)
-      // = -log2(e)/2 r^2 + pB r + pC
-      const float arg = __builtin_fmaf(pB, rr, __builtin_fmaf(pmf_a, r2, pC));
+      // -x = log2(e)/2 r^2 + pB r + pC
+      const float nx = __builtin_fmaf(pB, rr, __builtin_fmaf(pmf_a, r2, pC));
       // distance-dependent electrostatics (:72-78), charge applied per atom:
       // el1 g = poly(r) for s = el1 r < 1, else 1/r
       const float poly = __builtin_fmaf(__builtin_fmaf(k.e3, rr, k.e2), r2, k.e0);
       g = rr < k.inv_el1 ? poly : ri;
-      epmf = __builtin_fmaf(pmf1, gd_rcp(1.0f + gd_exp2(arg)), epmf);
-    } else {
-      epmf += pmf1;
+      const float t = gd_rcp(1.0f + gd_exp2(nx));
+      epmf = __builtin_fmaf(-pmf1, near ? t : 0.0f, epmf);
     }
     ele[j] = __builtin_fmaf(pele, g, ele[j]);
     // pocket-specific potential (:100-107): r <= pmf0 decided on r^2
@@ -207,27 +215,38 @@ __device__ __forceinline__ void gd_pair_class(
   const GD_AS_CONST char *__restrict__ ptab = (const GD_AS_CONST char *)gd_c(J.ptab);
   const GD_AS_CONST float *__restrict__ pspv = gd_c(J.pspv);
   const int p0 = lo + b;
+  if (p0 >= hi) return;
   // The point attributes (uniform) go through the vector memory path and are
   // prefetched one point ahead with the coordinates: as scalar loads of strided
-  // records they missed the scalar cache and stalled the wave.  The empty asm
-  // hides the index's uniformity, which would otherwise turn the load scalar.
-  int pv = p0 < hi ? p0 : 0;
-  asm("" : "+v"(pv));
-  float4 cn = pc[p0 < hi ? p0 : 0];
-  float4 an = pattr[pv];
-  for (int p = p0; p < hi; p += nP) {
-    const float4 c = cn, pa = an;
-    pv += nP;
-    if (p + nP < hi) {
-      cn = pc[p + nP];
-      an = pattr[pv];
-    }
+  // records they missed the scalar cache and stalled the wave.  The address is
+  // uniform (SGPRs) plus a zero VGPR offset whose value the compiler cannot
+  // see, which keeps the load on the vector path at no per-point VALU cost.
+  int zo = 0;
+  asm("" : "+v"(zo));
+  auto attr = [&](int q) {
+    return *(const GD_AS_GLOBAL float4 *)((const GD_AS_GLOBAL char *)(pattr + q) + zo);
+  };
+  auto point = [&](int q, const float4 c, const float4 pa) {
     const int fl = __builtin_amdgcn_readfirstlane(__float_as_int(pa.z));
     // byte offset of the point type's entry inside a ligand type's row block
     const unsigned poff = (unsigned)(fl & GD_PF_TYPE) * (GD_PTAB_W * 4);
-    gd_pair_atoms<LC, CLS>(k, ptab, pspv, P, p, poff, c, pa.x, pa.y, pmf_a, lx, ly, lz, toff, ele, hpc,
+    gd_pair_atoms<LC, CLS>(k, ptab, pspv, P, q, poff, c, pa.x, pa.y, pmf_a, lx, ly, lz, toff, ele, hpc,
                            evdw, epmf, epsp, ehdb);
+  };
+  // Two points per iteration, each prefetching the next point into the other
+  // register set, so the loop carries no register copies.  The prefetch index
+  // is clamped into the class (the last one reloads a point, unused).
+  float4 c0 = pc[p0], a0 = attr(p0);
+  int p = p0;
+  for (; p + nP < hi; p += 2 * nP) {
+    const float4 c1 = pc[p + nP], a1 = attr(p + nP);
+    point(p, c0, a0);
+    const int q = min(p + 2 * nP, hi - 1);
+    c0 = pc[q];
+    a0 = attr(q);
+    point(p + nP, c1, a1);
   }
+  if (p < hi) point(p, c0, a0);
 }
 
 // The point loop of one item, class by class (padding atoms of a partial chunk
@@ -238,9 +257,9 @@ __device__ __forceinline__ void gd_pair_points(
     const GdPairConst &k, const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC],
     const unsigned (&toff)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
     float &epsp, float &ehdb) {
-  // -log2(e)/2 held in a VGPR: a VALU op reads at most one SGPR or literal
+  // log2(e)/2 held in a VGPR: a VALU op reads at most one SGPR or literal
   float pmf_a;
-  asm("v_mov_b32 %0, 0xbf38aa3b" : "=v"(pmf_a));
+  asm("v_mov_b32 %0, 0x3f38aa3b" : "=v"(pmf_a));  // log2(e) / 2
   gd_pair_class<LC, 0>(J, J.pcls[0], J.pcls[1], b, pc, k, pmf_a, lx, ly, lz, toff, ele, hpc, evdw, epmf,
                        epsp, ehdb);
   gd_pair_class<LC, 1>(J, J.pcls[1], J.pcls[2], b, pc, k, pmf_a, lx, ly, lz, toff, ele, hpc, evdw, epmf,
@@ -618,7 +637,7 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
     float e[10];
     e[0] = s[0];
     e[1] = s[1];
-    e[2] = s[2];
+    e[2] = J.pmf_all + s[2];
     e[3] = s[3];
     e[4] = s[4] * (-1.0f / sqrtf(2.0f * 3.1415926535f));
     e[5] = s[6];

@@ -259,8 +259,10 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
   const int Ls = (L + 15) / 16 * 16;
   g.Ls = Ls;
   std::vector<float> ptab((size_t)(nty + 1) * GD_PTAB_PT * GD_PTAB_W, 0.0f);
-  for (int pt = 0; pt < GD_PTAB_PT; ++pt)
+  for (int pt = 0; pt < GD_PTAB_PT; ++pt) {
+    ptab[((size_t)nty * GD_PTAB_PT + pt) * GD_PTAB_W + 0] = std::numeric_limits<float>::infinity();
     ptab[((size_t)nty * GD_PTAB_PT + pt) * GD_PTAB_W + 5] = std::numeric_limits<float>::quiet_NaN();
+  }
   const double log2e = 1.44269504088896340736;
   const double hdb_c = std::sqrt(0.5 * log2e);
   for (int ti = 0; ti < nty; ++ti) {
@@ -268,11 +270,18 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
     for (int pt = 0; pt < GD_NPT; ++pt) {
       float *o = &ptab[((size_t)ti * GD_PTAB_PT + pt) * GD_PTAB_W];
       const double pmf0 = e->pmf0[lt][pt];
-      o[0] = e->p1a[lt][pt];
-      o[1] = -e->p2a[lt][pt];
-      // PMF exponent (6 - r/2)(r - pmf0) log2(e) = -log2(e)/2 r^2 + B r + C
-      o[2] = (float)(log2e * (6.0 + 0.5 * pmf0));
-      o[3] = (float)(-6.0 * log2e * pmf0);
+      // LJ over p1a: w = 1 / p1a and -q = -p2a / p1a; a row without LJ
+      // (p1a = p2a = 0) has w = inf, which makes the term 0
+      const double p1a = e->p1a[lt][pt], p2a = e->p2a[lt][pt];
+      if (p1a == 0.0 && p2a != 0.0) {
+        set_error("LJ parameters with p1a = 0 and p2a != 0 are not supported");
+        return GD_ERR_ARG;
+      }
+      o[0] = p1a != 0.0 ? (float)(1.0 / p1a) : std::numeric_limits<float>::infinity();
+      o[1] = p1a != 0.0 ? (float)(-p2a / p1a) : 0.0f;
+      // negated PMF exponent -(6 - r/2)(r - pmf0) log2(e) = log2(e)/2 r^2 + B r + C
+      o[2] = (float)(-log2e * (6.0 + 0.5 * pmf0));
+      o[3] = (float)(6.0 * log2e * pmf0);
       o[4] = e->pmf1[lt][pt];
       // psp cutoff on r^2 (compared against |o[5]|; NaN: never); the sign bit
       // marks the pairs whose H-bond term is on
@@ -371,6 +380,14 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       const size_t src = (size_t)i * P + porder[q];
       pco[(size_t)i * P + q] = make_float4(d->prt_x[src], d->prt_y[src], d->prt_z[src], 0.0f);
     }
+  // pmf1 summed over every (atom, point) pair: the PMF term of a pose is this
+  // constant minus the pair kernel's corrections
+  {
+    double sp = 0.0;
+    for (int l = 0; l < L; ++l)
+      for (int p = 0; p < P; ++p) sp += e->pmf1[d->lig_t[l]][d->prt_t[p]];
+    g.pmf_all = (float)sp;
+  }
   // scalars
   g.lj0 = e->lj0, g.lj1 = e->lj1, g.el0 = e->el0, g.el1 = e->el1, g.a1 = e->a1, g.b1 = e->b1;
   g.inv_el1 = 1.0f / e->el1;
