@@ -588,7 +588,11 @@ __global__ __launch_bounds__(256) void gd_reduce_kernel(const GdJob *__restrict_
 // ---------------------------------------------------------------------------
 // Finish kernel: 1024 threads = 16 waves per 64-replica block; lanes are
 // replicas.  Sums the reduce kernel's slices, assembles and weights the terms,
-// records, accepts, proposes the next move and transforms the atoms.
+// records, accepts, proposes the next move and transforms the atoms.  Only the
+// acceptance waits for the energies: the next step's increments, this step's
+// accept draw and the rotations of both possible next proposals (from the
+// proposal if it is accepted, from the current pose if not) are computed by
+// waves 1-3 while the slices are summed, and wave 0 only selects.
 #define GD_FW 16
 __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict__ jobs,
                                                           const GdBlock *__restrict__ blocks,
@@ -598,13 +602,17 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
   if (wq && blockIdx.x == 0 && threadIdx.x == 0) *wq = 0;  // next pair launch's queue
   if (step_base) launch_step += *step_base;  // graph replays: base step in device memory
   __shared__ float red[GD_FW][8][64];
-  __shared__ float xf[12][64];
+  __shared__ float xf[2][12][64];  // next transform if this step is accepted [0] / rejected [1]
+  __shared__ float inc_s[6][64];
+  __shared__ float u_s[64];
+  __shared__ int acc_s[64];
   const GdBlock bk = gd_c(blocks)[blockIdx.x];
   GdJobC &J = gd_c(jobs)[bk.job];
   const int lane = threadIdx.x & 63, wave = gd_uniform((int)(threadIdx.x >> 6));
   const int r = bk.rb + lane;
   const int Rp = J.Rp;
   const int step = launch_step - J.step0;  // this job's MC step
+  const bool move = mode != GD_MODE_EVAL;
 
   // (1) the reduce kernel's slices: wave w reads slices w, w + 16, ...
   {
@@ -618,8 +626,41 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
 #pragma unroll
     for (int c = 0; c < 8; ++c) red[wave][c][lane] = acc[c];
   }
+  // (2) off the acceptance path: wave 1 the next step's increments, wave 2 this
+  // step's accept draw (the initial pass draws u = 0, a forced accept)
+  const uint32_t next = mode == GD_MODE_STEP ? (uint32_t)step + 1u : 0u;
+  if (move && wave == 1) {
+    float inc[6], un;
+    gd_move(J, r, next, inc, &un);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) inc_s[i][lane] = inc[i];
+  }
+  if (move && wave == 2)
+    u_s[lane] = mode == GD_MODE_STEP
+                    ? gd_accept_u01(((uint64_t)J.seed_hi << 32) | J.seed_lo, J.job_id, (uint32_t)r,
+                                    (uint32_t)step)
+                    : 0.0f;
+  // the two possible bases of the next proposal, read before wave 0 writes them
+  float base[6];
+  if (move && (wave == 1 || wave == 3)) {
+    const GD_AS_GLOBAL float *src = gd_g(wave == 1 ? J.prop : J.mm);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) base[i] = src[(size_t)i * Rp + r];
+  }
   __syncthreads();
 
+  if (move && (wave == 1 || wave == 3)) {  // rotation of each candidate next proposal
+    float m[6], R[9];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) m[i] = inc_s[i][lane] + base[i];
+    gd_rotation(m, R);
+    const int c = wave == 1 ? 0 : 1;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) xf[c][i][lane] = R[i];
+    xf[c][9][lane] = m[0] + J.center[0];
+    xf[c][10][lane] = m[1] + J.center[1];
+    xf[c][11][lane] = m[2] + J.center[2];
+  }
   if (wave == 0) {
     GD_AS_GLOBAL float *prop = gd_g(J.prop), *en = gd_g(J.en), *mm = gd_g(J.mm);
     float s[8];
@@ -667,7 +708,7 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
     e[9] = tot;
 
     const bool live = r < J.R;
-    if (mode == GD_MODE_EVAL) {
+    if (!move) {
       if (live) {
         GD_AS_GLOBAL float *evt = gd_g(J.ev_terms), *evr = gd_g(J.ev_raw);
         for (int i = 0; i < 10; ++i) evt[(size_t)r * 10 + i] = e[i];
@@ -688,13 +729,10 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
         }
       }
       // Metropolis (:382-392); the initial pass draws u = 0
-      float u = 0.0f;
-      if (mode == GD_MODE_STEP) {
-        float inc[6];
-        gd_move(J, r, (uint32_t)step, inc, &u);
-      }
+      const float u = u_s[lane];
       const float x = (tot - en[(size_t)9 * Rp + r]) * gd_g(J.minus_beta)[gd_g(J.tmpi)[r]];
       const bool a = (u == 0.0f) ? (x >= GD_EXP_UNDERFLOW) : (u < expf(x));
+      acc_s[lane] = live && a;  // padding lanes keep their pose
       if (live) {
         acc_r[r] = a;
         if (a) {
@@ -705,33 +743,20 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
           best[r] = fminf(best[r], tot);
         }
       }
-      // propose and transform the next step's move
-      const uint32_t next = mode == GD_MODE_STEP ? (uint32_t)step + 1u : 0u;
-      float inc[6], un;
-      gd_move(J, r, next, inc, &un);
+      // the next step's proposal, from the accepted pose
 #pragma unroll
-      for (int i = 0; i < 6; ++i) {
-        const float v = inc[i] + mm[(size_t)i * Rp + r];
-        prop[(size_t)i * Rp + r] = v;
-        m[i] = v;
-      }
-      float R[9];
-      gd_rotation(m, R);
-#pragma unroll
-      for (int i = 0; i < 9; ++i) xf[i][lane] = R[i];
-      xf[9][lane] = m[0] + J.center[0];
-      xf[10][lane] = m[1] + J.center[1];
-      xf[11][lane] = m[2] + J.center[2];
+      for (int i = 0; i < 6; ++i) prop[(size_t)i * Rp + r] = inc_s[i][lane] + mm[(size_t)i * Rp + r];
     }
   }
-  if (mode == GD_MODE_EVAL) return;
+  if (!move) return;
   __syncthreads();
+  const int c = acc_s[lane] ? 0 : 1;
   float R[9], t[3];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) R[i] = xf[i][lane];
-  t[0] = xf[9][lane];
-  t[1] = xf[10][lane];
-  t[2] = xf[11][lane];
+  for (int i = 0; i < 9; ++i) R[i] = xf[c][i][lane];
+  t[0] = xf[c][9][lane];
+  t[1] = xf[c][10][lane];
+  t[2] = xf[c][11][lane];
   gd_transform_atoms(J, r, R, t, wave, GD_FW);
 }
 

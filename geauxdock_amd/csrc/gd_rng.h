@@ -79,6 +79,12 @@ GD_HD void gd_step_draws(uint64_t seed, uint32_t job, uint32_t replica, uint32_t
   w[7] = b.v[3];
 }
 
+// Accept draw of a replica-step alone (word 6 of gd_step_draws: slot 1 only).
+GD_HD float gd_accept_u01(uint64_t seed, uint32_t job, uint32_t replica, uint32_t step) {
+  const gd_u32x4 b = gd_philox4x32(step, replica, job, 1u, (uint32_t)seed, (uint32_t)(seed >> 32));
+  return gd_u01(b.v[2]);
+}
+
 // Exchange draw for pair `pair` in exchange round `round`.
 GD_HD float gd_exchange_u01(uint64_t seed, uint32_t job, uint32_t pair, uint32_t round) {
   const gd_u32x4 a = gd_philox4x32(round, pair, job, 2u, (uint32_t)seed, (uint32_t)(seed >> 32));
