@@ -24,6 +24,13 @@ struct gd_job {
   GdItem *d_items = nullptr;
   GdBlock *d_blocks = nullptr;
   int n_items = 0, n_blocks = 0;
+  // protein points in source order, laid out on the device per tile count
+  std::vector<float4> src_pco;     // [n_prt][P]
+  std::vector<float4> src_pattr;   // [P]
+  std::vector<float> src_pspv;     // [n_types + 1][P]
+  std::vector<int> src_key;        // [P] class * GD_PTAB_PT + point type
+  std::vector<int> src_spread;     // [P] order inside a (class, type) group
+  int laid_nP = 0;                 // tile count of the device order (0: none yet)
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

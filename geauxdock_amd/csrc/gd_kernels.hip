@@ -84,44 +84,24 @@ struct GdPairConst {
 
 #define GD_ROW 9  // force-field row entries used by the pair loop
 
-// The LC chunk atoms against one point.  CLS: bit 0 = cdc point (hydrophobic
-// term), bit 1 = class-2 point (psp term); the caller dispatches on the point's
-// flags once, so the atoms carry no per-pair class tests.
-template <int LC, int CLS>
-__device__ __forceinline__ void gd_pair_atoms(
-    const GdPairConst &k, const GD_AS_CONST char *__restrict__ ptab,
-    const GD_AS_CONST float *__restrict__ pspv, const int P, const int p, const unsigned poff,
-    const float4 c, const float pele, const float phpp, const float pmf_a, const float (&lx)[LC],
-    const float (&ly)[LC], const float (&lz)[LC], const unsigned (&toff)[LC], float (&ele)[LC],
-    float (&hpc)[LC], float &evdw, float &epmf, float &epsp, float &ehdb) {
-  // Force-field rows are software-pipelined: the row of atom j + 1 is loaded
-  // while atom j is evaluated.  Scalar loads complete out of order, so every
-  // wait is for all of them; the asm with a memory clobber places the wait for
-  // row j before the load of row j + 1 is issued.
-  float rw[GD_ROW];
-  {
-    const GD_AS_CONST float *__restrict__ prm = (const GD_AS_CONST float *)(ptab + (toff[0] + poff));
-#pragma unroll
-    for (int i = 0; i < GD_ROW; ++i) rw[i] = prm[i];
-  }
-#pragma unroll
-  for (int j = 0; j < LC; ++j) {
-    asm volatile("" ::"s"(rw[0]), "s"(rw[1]), "s"(rw[2]), "s"(rw[3]), "s"(rw[4]), "s"(rw[5]),
-                 "s"(rw[6]), "s"(rw[7]), "s"(rw[8])
-                 : "memory");
+// One (atom, point) pair: the six terms of energy_prt_v1.C:30-134 for one
+// ligand atom of the lane against one protein point, with the force-field row
+// rw of (atom type, point type) in scalar registers.  CLS: bit 0 = cdc point
+// (hydrophobic term), bit 1 = class-2 point (psp term); the caller dispatches
+// on the point's flags once, so the pairs carry no per-pair class tests.
+template <int CLS>
+__device__ __forceinline__ void gd_pair_one(
+    const GdPairConst &k, const float (&rw)[GD_ROW], const GD_AS_CONST float *__restrict__ pspv,
+    const int P, const int p, const unsigned prow, const float4 c, const float pele, const float phpp,
+    const float pmf_a, const float lxj, const float lyj, const float lzj, float &elej, float &hpcj,
+    float &evdw, float &epmf, float &epsp, float &ehdb) {
     const float w9 = rw[0], nq = rw[1], pB = rw[2], pC = rw[3];
     const float pmf1 = rw[4], pthr = rw[5], hdb1c = rw[6], nhdb0c = rw[7], r2far = rw[8];
-    if (j + 1 < LC) {
-      const GD_AS_CONST float *__restrict__ prm =
-          (const GD_AS_CONST float *)(ptab + (toff[j + 1 < LC ? j + 1 : j] + poff));
-#pragma unroll
-      for (int i = 0; i < GD_ROW; ++i) rw[i] = prm[i];
-    }
     // the sign bit of the psp threshold flags an active H-bond pair; the asm
     // keeps the test on the scalar unit (as a float compare it became a VALU op)
     int hflag;
     asm("" : "=s"(hflag) : "0"(__float_as_int(pthr)));
-    const float dx = lx[j] - c.x, dy = ly[j] - c.y, dz = lz[j] - c.z;
+    const float dx = lxj - c.x, dy = lyj - c.y, dz = lzj - c.z;
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     const float ri = gd_rsq(r2);
     const float rr = r2 * ri;
@@ -136,7 +116,7 @@ __device__ __forceinline__ void gd_pair_atoms(
         float w = __builtin_fmaf(-GD_HPD, r2, GD_HPC);
         w = __builtin_fmaf(w, r2, -GD_HPB);
         w = __builtin_fmaf(w, r2, GD_HPA);
-        hpc[j] = __builtin_fmaf(phpp, __builtin_fmaf(-r2, w, 1.0f), hpc[j]);
+        hpcj = __builtin_fmaf(phpp, __builtin_fmaf(-r2, w, 1.0f), hpcj);
       }
     }
     // soft Lennard-Jones (:61-64): (p1 - p2) / (p1 lj0 (1 + lj1 r^2) + 1) with
@@ -175,15 +155,14 @@ __device__ __forceinline__ void gd_pair_atoms(
       const float t = gd_rcp(1.0f + gd_exp2(nx));
       epmf = __builtin_fmaf(-pmf1, near ? t : 0.0f, epmf);
     }
-    ele[j] = __builtin_fmaf(pele, g, ele[j]);
+    elej = __builtin_fmaf(pele, g, elej);
     // pocket-specific potential (:100-107): r <= pmf0 decided on r^2
     // against the exact float threshold of sqrtf(r^2) <= pmf0; the value is
     // loaded only when some lane is in contact (exec-masked branch)
     if (CLS & 2) {
       if (r2 <= fabsf(pthr)) {
         asm volatile("");
-        const unsigned row = toff[j] / (GD_PTAB_PT * GD_PTAB_W * 4);
-        epsp += pspv[(size_t)row * P + p];
+        epsp += pspv[(size_t)prow * P + p];
       }
     }
     // hydrogen bond (:116-121) as c exp2(-(r c - hdb0 c)^2), c = hdb1 sqrt(log2(e)/2),
@@ -194,44 +173,129 @@ __device__ __forceinline__ void gd_pair_atoms(
       const float h = __builtin_fmaf(rr, hdb1c, nhdb0c);
       ehdb = __builtin_fmaf(hdb1c, gd_exp2(-(h * h)), ehdb);
     }
-    // one atom at a time: interleaving atoms raises register use past the
-    // scalar register file (spills) for no gain in issue rate
+}
+
+// Rows of the LC chunk atoms for one point type, loaded into scalar registers.
+template <int LC>
+__device__ __forceinline__ void gd_load_rows(const GD_AS_CONST char *__restrict__ ptab,
+                                             const unsigned (&toff)[LC], const unsigned poff,
+                                             float (&rw)[LC][GD_ROW]) {
+#pragma unroll
+  for (int j = 0; j < LC; ++j) {
+    const GD_AS_CONST float *__restrict__ prm = (const GD_AS_CONST float *)(ptab + (toff[j] + poff));
+#pragma unroll
+    for (int i = 0; i < GD_ROW; ++i) rw[j][i] = prm[i];
+  }
+}
+
+// The LC chunk atoms against one point, rows given (held across the points of
+// one point type).
+template <int LC, int CLS>
+__device__ __forceinline__ void gd_pair_atoms_rows(
+    const GdPairConst &k, const float (&rw)[LC][GD_ROW], const GD_AS_CONST float *__restrict__ pspv,
+    const int P, const int p, const float4 c, const float pele, const float phpp, const float pmf_a,
+    const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC], const unsigned (&trow)[LC],
+    float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf, float &epsp, float &ehdb) {
+#pragma unroll
+  for (int j = 0; j < LC; ++j) {
+    gd_pair_one<CLS>(k, rw[j], pspv, P, p, trow[j], c, pele, phpp, pmf_a, lx[j], ly[j], lz[j], ele[j],
+                     hpc[j], evdw, epmf, epsp, ehdb);
+    // one atom at a time: interleaving atoms raises register use for no gain
+    // in issue rate
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// The LC chunk atoms against one point, each atom's row loaded for this point
+// (wide chunks, whose rows do not fit the scalar registers).  The loads are
+// software-pipelined: the row of atom j + 1 is loaded while atom j is
+// evaluated.  Scalar loads complete out of order, so every wait is for all of
+// them; the asm with a memory clobber places the wait for row j before the
+// load of row j + 1 is issued.
+template <int LC, int CLS>
+__device__ __forceinline__ void gd_pair_atoms_load(
+    const GdPairConst &k, const GD_AS_CONST char *__restrict__ ptab,
+    const GD_AS_CONST float *__restrict__ pspv, const int P, const int p, const unsigned poff,
+    const float4 c, const float pele, const float phpp, const float pmf_a, const float (&lx)[LC],
+    const float (&ly)[LC], const float (&lz)[LC], const unsigned (&toff)[LC],
+    const unsigned (&trow)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
+    float &epsp, float &ehdb) {
+  float rw[GD_ROW];
+  {
+    const GD_AS_CONST float *__restrict__ prm = (const GD_AS_CONST float *)(ptab + (toff[0] + poff));
+#pragma unroll
+    for (int i = 0; i < GD_ROW; ++i) rw[i] = prm[i];
+  }
+#pragma unroll
+  for (int j = 0; j < LC; ++j) {
+    asm volatile("" ::"s"(rw[0]), "s"(rw[1]), "s"(rw[2]), "s"(rw[3]), "s"(rw[4]), "s"(rw[5]),
+                 "s"(rw[6]), "s"(rw[7]), "s"(rw[8])
+                 : "memory");
+    float cur[GD_ROW];
+#pragma unroll
+    for (int i = 0; i < GD_ROW; ++i) cur[i] = rw[i];
+    if (j + 1 < LC) {
+      const GD_AS_CONST float *__restrict__ prm =
+          (const GD_AS_CONST float *)(ptab + (toff[j + 1 < LC ? j + 1 : j] + poff));
+#pragma unroll
+      for (int i = 0; i < GD_ROW; ++i) rw[i] = prm[i];
+    }
+    gd_pair_one<CLS>(k, cur, pspv, P, p, trow[j], c, pele, phpp, pmf_a, lx[j], ly[j], lz[j], ele[j],
+                     hpc[j], evdw, epmf, epsp, ehdb);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
 
 // The points of one class [lo, hi) of one tile against the LC chunk atoms:
-// tile b of nP takes lo + b, lo + b + nP, ...  Every tile gets the same share of
-// every class and the same mix of points near and far from the pocket, so
-// waves cost about the same.
+// tile b of nP takes lo + b, lo + b + nP, ...  The host lays each class out so
+// that these are a contiguous piece of the class's points sorted by point
+// type, so the force-field rows of a 4-atom chunk change only where the point
+// type does and stay in scalar registers across the run; within a type the
+// order spreads the distances to the pocket evenly over the tiles, so every
+// tile holds the same mix of near and far points and waves cost the same.
 template <int LC, int CLS>
 __device__ __forceinline__ void gd_pair_class(
     const GD_AS_CONST GdJob &J, const int lo, const int hi, const int b,
     const GD_AS_GLOBAL float4 *__restrict__ pc, const GdPairConst &k, const float pmf_a,
     const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC], const unsigned (&toff)[LC],
-    float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf, float &epsp, float &ehdb) {
+    const unsigned (&trow)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
+    float &epsp, float &ehdb) {
   const int P = J.P, nP = J.nP;
+  const int p0 = lo + b;
+  if (p0 >= hi) return;
   const GD_AS_GLOBAL float4 *__restrict__ pattr = gd_g(J.pattr);
   const GD_AS_CONST char *__restrict__ ptab = (const GD_AS_CONST char *)gd_c(J.ptab);
   const GD_AS_CONST float *__restrict__ pspv = gd_c(J.pspv);
-  const int p0 = lo + b;
-  if (p0 >= hi) return;
   // The point attributes (uniform) go through the vector memory path and are
-  // prefetched one point ahead with the coordinates: as scalar loads of strided
-  // records they missed the scalar cache and stalled the wave.  The address is
-  // uniform (SGPRs) plus a zero VGPR offset whose value the compiler cannot
-  // see, which keeps the load on the vector path at no per-point VALU cost.
+  // prefetched one point ahead with the coordinates: as scalar loads they
+  // stalled the wave.  The address is uniform (SGPRs) plus a zero VGPR offset
+  // whose value the compiler cannot see, which keeps the load on the vector
+  // path at no per-point VALU cost.
   int zo = 0;
   asm("" : "+v"(zo));
   auto attr = [&](int q) {
     return *(const GD_AS_GLOBAL float4 *)((const GD_AS_GLOBAL char *)(pattr + q) + zo);
   };
+  constexpr bool kRows = LC == 4;  // rows of 4 atoms fit the scalar registers
+  float rw[kRows ? LC : 1][GD_ROW];
+  int ty_cur = -1;
   auto point = [&](int q, const float4 c, const float4 pa) {
     const int fl = __builtin_amdgcn_readfirstlane(__float_as_int(pa.z));
+    const int ty = fl & GD_PF_TYPE;
     // byte offset of the point type's entry inside a ligand type's row block
-    const unsigned poff = (unsigned)(fl & GD_PF_TYPE) * (GD_PTAB_W * 4);
-    gd_pair_atoms<LC, CLS>(k, ptab, pspv, P, q, poff, c, pa.x, pa.y, pmf_a, lx, ly, lz, toff, ele, hpc,
-                           evdw, epmf, epsp, ehdb);
+    const unsigned poff = (unsigned)ty * (GD_PTAB_W * 4);
+    if constexpr (kRows) {
+      if (ty != ty_cur) {
+        ty_cur = ty;
+        gd_load_rows<LC>(ptab, toff, poff, *reinterpret_cast<float(*)[LC][GD_ROW]>(&rw));
+      }
+      gd_pair_atoms_rows<LC, CLS>(k, *reinterpret_cast<float(*)[LC][GD_ROW]>(&rw), pspv, P, q, c,
+                                  pa.x, pa.y, pmf_a, lx, ly, lz, trow, ele, hpc, evdw, epmf, epsp,
+                                  ehdb);
+    } else {
+      gd_pair_atoms_load<LC, CLS>(k, ptab, pspv, P, q, poff, c, pa.x, pa.y, pmf_a, lx, ly, lz, toff,
+                                  trow, ele, hpc, evdw, epmf, epsp, ehdb);
+    }
   };
   // Two points per iteration, each prefetching the next point into the other
   // register set, so the loop carries no register copies.  The prefetch index
@@ -249,25 +313,25 @@ __device__ __forceinline__ void gd_pair_class(
   if (p < hi) point(p, c0, a0);
 }
 
-// The point loop of one item, class by class (padding atoms of a partial chunk
-// contribute exactly zero).
+// The point loop of one item, class by class (padding atoms of a partial
+// chunk contribute exactly zero).
 template <int LC>
 __device__ __forceinline__ void gd_pair_points(
     const GD_AS_CONST GdJob &J, const int b, const GD_AS_GLOBAL float4 *__restrict__ pc,
     const GdPairConst &k, const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC],
-    const unsigned (&toff)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
-    float &epsp, float &ehdb) {
+    const unsigned (&toff)[LC], const unsigned (&trow)[LC], float (&ele)[LC], float (&hpc)[LC],
+    float &evdw, float &epmf, float &epsp, float &ehdb) {
   // log2(e)/2 held in a VGPR: a VALU op reads at most one SGPR or literal
   float pmf_a;
   asm("v_mov_b32 %0, 0x3f38aa3b" : "=v"(pmf_a));  // log2(e) / 2
-  gd_pair_class<LC, 0>(J, J.pcls[0], J.pcls[1], b, pc, k, pmf_a, lx, ly, lz, toff, ele, hpc, evdw, epmf,
-                       epsp, ehdb);
-  gd_pair_class<LC, 1>(J, J.pcls[1], J.pcls[2], b, pc, k, pmf_a, lx, ly, lz, toff, ele, hpc, evdw, epmf,
-                       epsp, ehdb);
-  gd_pair_class<LC, 2>(J, J.pcls[2], J.pcls[3], b, pc, k, pmf_a, lx, ly, lz, toff, ele, hpc, evdw, epmf,
-                       epsp, ehdb);
-  gd_pair_class<LC, 3>(J, J.pcls[3], J.pcls[4], b, pc, k, pmf_a, lx, ly, lz, toff, ele, hpc, evdw, epmf,
-                       epsp, ehdb);
+  gd_pair_class<LC, 0>(J, J.pcls[0], J.pcls[1], b, pc, k, pmf_a, lx, ly, lz, toff, trow, ele, hpc,
+                       evdw, epmf, epsp, ehdb);
+  gd_pair_class<LC, 1>(J, J.pcls[1], J.pcls[2], b, pc, k, pmf_a, lx, ly, lz, toff, trow, ele, hpc,
+                       evdw, epmf, epsp, ehdb);
+  gd_pair_class<LC, 2>(J, J.pcls[2], J.pcls[3], b, pc, k, pmf_a, lx, ly, lz, toff, trow, ele, hpc,
+                       evdw, epmf, epsp, ehdb);
+  gd_pair_class<LC, 3>(J, J.pcls[3], J.pcls[4], b, pc, k, pmf_a, lx, ly, lz, toff, trow, ele, hpc,
+                       evdw, epmf, epsp, ehdb);
 }
 
 template <int LC>
@@ -285,14 +349,15 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
 
   // ligand chunk: coordinates of this lane's replica, per-atom constants (uniform)
   float lx[LC], ly[LC], lz[LC], ele[LC], hpc[LC];
-  unsigned toff[LC];
+  unsigned toff[LC], trow[LC];  // row block of each atom: byte offset, index
 #pragma unroll
   for (int j = 0; j < LC; ++j) {
     const int l = l0 + min(j, nl - 1);
     lx[j] = coords[(size_t)(3 * l + 0) * Rp + r];
     ly[j] = coords[(size_t)(3 * l + 1) * Rp + r];
     lz[j] = coords[(size_t)(3 * l + 2) * Rp + r];
-    toff[j] = (unsigned)atoms[l0 + j].tidx * (GD_PTAB_PT * GD_PTAB_W * 4);
+    trow[j] = (unsigned)atoms[l0 + j].tidx;
+    toff[j] = trow[j] * (GD_PTAB_PT * GD_PTAB_W * 4);
     ele[j] = 0.0f;
     hpc[j] = 0.0f;
   }
@@ -307,7 +372,7 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   k.inv_el1 = J.inv_el1;
   const float inv_el1 = k.inv_el1;
   float evdw = 0.0f, epmf = 0.0f, epsp = 0.0f, ehdb = 0.0f;
-  gd_pair_points<LC>(J, b, pc, k, lx, ly, lz, toff, ele, hpc, evdw, epmf, epsp, ehdb);
+  gd_pair_points<LC>(J, b, pc, k, lx, ly, lz, toff, trow, ele, hpc, evdw, epmf, epsp, ehdb);
   ehdb *= GD_HDB_RC;
 
   // KDE: this tile's share of every chunk atom's anchor bin, a contiguous range

@@ -193,6 +193,57 @@ static gd_status alloc_partials(gd_job *j, GdJob &g, int cap_rp) {
   return a.commit(&j->d_part);
 }
 
+// Device order of the protein points for the job's tile count: class by
+// class, tile b of nP reads points pcls[c] + b, + nP, ... (strided, see the
+// pair kernel).  The class's points sorted by (type, spread) are dealt out so
+// that each tile's points are one contiguous piece of that list: few type
+// changes per tile (the pair kernel keeps a chunk's force-field rows across a
+// type), and, by the spread order, a representative mix of distances.  The
+// order changes only summation order.
+static gd_status layout_points(gd_job *j) {
+  GdJob &g = j->h;
+  const int P = g.P, nP = g.nP, np = g.n_prt;
+  std::vector<int> ord(P), porder(P);
+  for (int p = 0; p < P; ++p) ord[p] = p;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+    return j->src_key[x] != j->src_key[y] ? j->src_key[x] < j->src_key[y]
+                                          : j->src_spread[x] < j->src_spread[y];
+  });
+  for (int c = 0; c < 4; ++c) {
+    const int lo = g.pcls[c], n = g.pcls[c + 1] - lo, q = n / nP, r = n % nP;
+    for (int i = 0; i < n; ++i) {
+      int b, k;  // column (tile) and row of list element i: columns are filled in turn
+      if (i < r * (q + 1)) {
+        b = i / (q + 1), k = i % (q + 1);
+      } else {
+        const int i2 = i - r * (q + 1);
+        b = r + i2 / q, k = i2 % q;
+      }
+      porder[lo + k * nP + b] = ord[lo + i];
+    }
+  }
+  const size_t nty1 = j->src_pspv.size() / (size_t)P;
+  std::vector<float4> pco((size_t)np * P), pattr(P);
+  std::vector<float> pspv(j->src_pspv.size());
+  for (int i = 0; i < np; ++i)
+    for (int q2 = 0; q2 < P; ++q2) pco[(size_t)i * P + q2] = j->src_pco[(size_t)i * P + porder[q2]];
+  for (int q2 = 0; q2 < P; ++q2) pattr[q2] = j->src_pattr[porder[q2]];
+  for (size_t t = 0; t < nty1; ++t)
+    for (int q2 = 0; q2 < P; ++q2) pspv[t * P + q2] = j->src_pspv[t * P + porder[q2]];
+  hipError_t e;
+  if ((e = hipStreamSynchronize(j->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync((void *)g.pcoord, pco.data(), pco.size() * sizeof(float4),
+                          hipMemcpyHostToDevice, j->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync((void *)g.pattr, pattr.data(), pattr.size() * sizeof(float4),
+                          hipMemcpyHostToDevice, j->stream)) != hipSuccess ||
+      (e = hipMemcpyAsync((void *)g.pspv, pspv.data(), pspv.size() * sizeof(float),
+                          hipMemcpyHostToDevice, j->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(j->stream)) != hipSuccess)
+    return hip_fail(e, "point layout upload");
+  j->laid_nP = nP;
+  return GD_OK;
+}
+
 gd_status apply_tiling(gd_job *j, int lc, int ptile) {
   GdJob &g = j->h;
   if (lc != 4 && lc != 8 && lc != 16) return GD_ERR_ARG;
@@ -200,13 +251,14 @@ gd_status apply_tiling(gd_job *j, int lc, int ptile) {
   g.lc = lc;
   g.nL = (g.L + lc - 1) / lc;
   g.ptile = ptile;
-  // protein tiles: points p = b, b + nP, ...; a multiple of GD_GROUP so that a
-  // work group's waves take tiles 4g .. 4g + 3 of the same chunk
+  // protein tiles: contiguous ranges of the device point order; a multiple of
+  // GD_GROUP so that a work group's waves take tiles 8g .. 8g + 7 of one chunk
   const int nP = (g.P + ptile - 1) / ptile;
   g.nP = (nP + GD_GROUP - 1) / GD_GROUP * GD_GROUP;
   g.nG = g.nP / GD_GROUP;
   gd_status st = alloc_partials(j, g, g.Rp);
   if (st != GD_OK) return st;
+  if (g.nP != j->laid_nP && (st = layout_points(j)) != GD_OK) return st;
   j->items.clear();  // one item per work group: (replica block, chunk, tile group)
   for (int rb = 0; rb < g.Rp; rb += 64)
     for (int gi = 0; gi < g.nG; ++gi)
@@ -296,32 +348,57 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       o[8] = far_r2_threshold(pmf0, e->el1);
     }
   }
-  // Protein points in device order: grouped by class (cdc flag + 2 x class-2
-  // flag; stable), so the pair kernel runs one loop per class with the class
-  // terms resolved at compile time.  Point order only changes summation order.
-  std::vector<int> porder(P);
+  // Protein points, kept in source order: the device order depends on the tile
+  // count and is laid out by layout_points (called from apply_tiling).
+  // Classes: cdc flag + 2 x class-2 flag.
   {
     int nc[4] = {0, 0, 0, 0};
     auto cls = [&](int p) { return (d->prt_cdc[p] == 1 ? 1 : 0) + (d->prt_c[p] == 2 ? 2 : 0); };
     for (int p = 0; p < P; ++p) ++nc[cls(p)];
     g.pcls[0] = 0;
     for (int c = 0; c < 4; ++c) g.pcls[c + 1] = g.pcls[c] + nc[c];
-    int fill[4] = {g.pcls[0], g.pcls[1], g.pcls[2], g.pcls[3]};
-    for (int p = 0; p < P; ++p) porder[fill[cls(p)]++] = p;
+    j->src_key.resize(P);
+    for (int p = 0; p < P; ++p) j->src_key[p] = cls(p) * GD_PTAB_PT + d->prt_t[p];
+    // within a (class, type) group: the bit-reversed rank of the distance to the
+    // pocket centre (protein conformer 0), a low-discrepancy order in which any
+    // run of consecutive points spans near and far points evenly
+    std::vector<float> dist(P);
+    for (int p = 0; p < P; ++p) {
+      const float dx = d->prt_x[p] - d->pocket_center[0], dy = d->prt_y[p] - d->pocket_center[1],
+                  dz = d->prt_z[p] - d->pocket_center[2];
+      dist[p] = dx * dx + dy * dy + dz * dz;
+    }
+    std::vector<int> ord(P);
+    for (int p = 0; p < P; ++p) ord[p] = p;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+      return j->src_key[x] != j->src_key[y] ? j->src_key[x] < j->src_key[y] : dist[x] < dist[y];
+    });
+    j->src_spread.resize(P);
+    for (int i = 0; i < P;) {
+      int e = i;
+      while (e < P && j->src_key[ord[e]] == j->src_key[ord[i]]) ++e;
+      int bits = 0;
+      while ((1 << bits) < e - i) ++bits;
+      for (int k = i; k < e; ++k) {
+        unsigned v = (unsigned)(k - i), rv = 0;
+        for (int t = 0; t < bits; ++t) rv |= ((v >> t) & 1u) << (bits - 1 - t);
+        j->src_spread[ord[k]] = (int)rv;
+      }
+      i = e;
+    }
   }
-  std::vector<float> pspv((size_t)(nty + 1) * P, 0.0f);  // last row: padding atoms (zero)
-  std::vector<float4> pattr(P);
-  for (int q = 0; q < P; ++q) {
-    const int p = porder[q];
+  j->src_pspv.assign((size_t)(nty + 1) * P, 0.0f);  // last row: padding atoms (zero)
+  j->src_pattr.resize(P);
+  for (int p = 0; p < P; ++p) {
     const int c = d->prt_c[p];
     int fl = d->prt_t[p] | (d->prt_cdc[p] == 1 ? GD_PF_CDC : 0) | (c == 2 ? GD_PF_CLS2 : 0);
     float flf;
     std::memcpy(&flf, &fl, 4);
-    pattr[q] = make_float4(d->prt_ele[p], d->prt_hpp[p], flf, 0.0f);
+    j->src_pattr[p] = make_float4(d->prt_ele[p], d->prt_hpp[p], flf, 0.0f);
     if (c == 2) {
       const int res = d->prt_seq3r[p];
       for (int ti = 0; ti < nty; ++ti)
-        pspv[(size_t)ti * P + q] =
+        j->src_pspv[(size_t)ti * P + p] =
             (res >= 0 && res < d->psp_ncol) ? d->psp[(size_t)types[ti] * d->psp_ncol + res] : 0.0f;
     }
   }
@@ -369,17 +446,15 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
     }
     mcsrow[m] = make_float2(d->mcs_tcc[m], (float)n);
   }
-  std::vector<float4> lig0((size_t)nl * L), pco((size_t)np * P);
+  std::vector<float4> lig0((size_t)nl * L);
   for (int i = 0; i < nl; ++i)
     for (int l = 0; l < L; ++l) {
       const size_t q = (size_t)i * L + l;
       lig0[q] = make_float4(d->lig_x[q], d->lig_y[q], d->lig_z[q], 0.0f);
     }
-  for (int i = 0; i < np; ++i)
-    for (int q = 0; q < P; ++q) {
-      const size_t src = (size_t)i * P + porder[q];
-      pco[(size_t)i * P + q] = make_float4(d->prt_x[src], d->prt_y[src], d->prt_z[src], 0.0f);
-    }
+  j->src_pco.resize((size_t)np * P);
+  for (size_t q = 0; q < (size_t)np * P; ++q)
+    j->src_pco[q] = make_float4(d->prt_x[q], d->prt_y[q], d->prt_z[q], 0.0f);
   // pmf1 summed over every (atom, point) pair: the PMF term of a pose is this
   // constant minus the pair kernel's corrections
   {
@@ -405,10 +480,10 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
   float2 *d_mcsrow;
   a.add(&d_lig0, lig0.size());
   a.add(&d_atoms, atoms.size());
-  a.add(&d_pco, pco.size());
-  a.add(&d_pattr, pattr.size());
+  a.add(&d_pco, j->src_pco.size());
+  a.add(&d_pattr, j->src_pattr.size());
   a.add(&d_ptab, ptab.size());
-  a.add(&d_pspv, pspv.size());
+  a.add(&d_pspv, j->src_pspv.size());
   a.add(&d_kde, kde.size());
   a.add(&d_mcs, mcs.size());
   a.add(&d_mcsrow, mcsrow.size());
@@ -442,10 +517,7 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
   };
   if ((st = up(d_lig0, lig0.data(), lig0.size() * sizeof(float4))) != GD_OK) return st;
   if ((st = up(d_atoms, atoms.data(), atoms.size() * sizeof(GdAtom))) != GD_OK) return st;
-  if ((st = up(d_pco, pco.data(), pco.size() * sizeof(float4))) != GD_OK) return st;
-  if ((st = up(d_pattr, pattr.data(), pattr.size() * sizeof(float4))) != GD_OK) return st;
   if ((st = up(d_ptab, ptab.data(), ptab.size() * sizeof(float))) != GD_OK) return st;
-  if ((st = up(d_pspv, pspv.data(), pspv.size() * sizeof(float))) != GD_OK) return st;
   if ((st = up(d_kde, kde.data(), kde.size() * sizeof(float4))) != GD_OK) return st;
   if ((st = up(d_mcs, mcs.data(), mcs.size() * sizeof(float4))) != GD_OK) return st;
   if ((st = up(d_mcsrow, mcsrow.data(), mcsrow.size() * sizeof(float2))) != GD_OK) return st;
