@@ -158,20 +158,21 @@ float far_r2_threshold(double pmf0, double el1) {
 
 // ---------------------------------------------------------------------------
 // Tiling: ligand chunk size lc (kernel template), protein tile ptile.  The
-// default aims for ~4 waves per SIMD over the whole launch; it depends only
+// default aims for ~22 waves per SIMD over the whole launch; it depends only
 // on the job so a job's reduction order is the same alone or in a batch.
 bool tuned_tiling(int L, int P, int R, int &lc, int &ptile);  // gd_batch.hip
 
 void default_tiling(int L, int P, int R, int &lc, int &ptile) {
   if (tuned_tiling(L, P, R, lc, ptile)) return;
-  // 4 atoms per wave keeps the pair kernel at ~55 VGPRs (8+ waves per SIMD);
-  // small point tiles give many equal-cost waves, and the group reduction keeps
-  // the partial-sum volume independent of how many tiles a group covers
+  // 4 atoms per wave keeps the pair kernel at ~62 VGPRs (8 waves per SIMD);
+  // ~22 waves per SIMD over the launch (about three waves per slot) balanced
+  // per-tile overhead (KDE and MCS shares, partial sums) against the launch's
+  // tail in a sweep of tile counts on a 3000-point, 1760-replica job
   lc = 4;
   const int nL = (L + lc - 1) / lc;
   const int nblk = (R + 63) / 64;
   const int simds = 4 * (g_cu > 0 ? g_cu : 256);
-  const int target_waves = 32 * simds;
+  const int target_waves = 22 * simds;
   int nP = std::max(1, target_waves / std::max(1, nblk * nL));
   nP = std::min(nP, std::max(1, P / 16));
   nP = (nP + GD_GROUP - 1) / GD_GROUP * GD_GROUP;
