@@ -83,6 +83,7 @@ struct GdPairConst {
 };
 
 #define GD_ROW 9  // force-field row entries used by the pair loop
+#define GD_ROW_BLOCK (GD_PTAB_PT * GD_PTAB_W * 4)  // bytes of one ligand type's row block
 
 // One (atom, point) pair: the six terms of energy_prt_v1.C:30-134 for one
 // ligand atom of the lane against one protein point, with the force-field row
@@ -178,11 +179,12 @@ __device__ __forceinline__ void gd_pair_one(
 // Rows of the LC chunk atoms for one point type, loaded into scalar registers.
 template <int LC>
 __device__ __forceinline__ void gd_load_rows(const GD_AS_CONST char *__restrict__ ptab,
-                                             const unsigned (&toff)[LC], const unsigned poff,
+                                             const unsigned (&trow)[LC], const unsigned poff,
                                              float (&rw)[LC][GD_ROW]) {
 #pragma unroll
   for (int j = 0; j < LC; ++j) {
-    const GD_AS_CONST float *__restrict__ prm = (const GD_AS_CONST float *)(ptab + (toff[j] + poff));
+    const GD_AS_CONST float *__restrict__ prm =
+        (const GD_AS_CONST float *)(ptab + (trow[j] * GD_ROW_BLOCK + poff));
 #pragma unroll
     for (int i = 0; i < GD_ROW; ++i) rw[j][i] = prm[i];
   }
@@ -217,12 +219,12 @@ __device__ __forceinline__ void gd_pair_atoms_load(
     const GdPairConst &k, const GD_AS_CONST char *__restrict__ ptab,
     const GD_AS_CONST float *__restrict__ pspv, const int P, const int p, const unsigned poff,
     const float4 c, const float pele, const float phpp, const float pmf_a, const float (&lx)[LC],
-    const float (&ly)[LC], const float (&lz)[LC], const unsigned (&toff)[LC],
-    const unsigned (&trow)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
-    float &epsp, float &ehdb) {
+    const float (&ly)[LC], const float (&lz)[LC], const unsigned (&trow)[LC], float (&ele)[LC],
+    float (&hpc)[LC], float &evdw, float &epmf, float &epsp, float &ehdb) {
   float rw[GD_ROW];
   {
-    const GD_AS_CONST float *__restrict__ prm = (const GD_AS_CONST float *)(ptab + (toff[0] + poff));
+    const GD_AS_CONST float *__restrict__ prm =
+        (const GD_AS_CONST float *)(ptab + (trow[0] * GD_ROW_BLOCK + poff));
 #pragma unroll
     for (int i = 0; i < GD_ROW; ++i) rw[i] = prm[i];
   }
@@ -236,7 +238,7 @@ __device__ __forceinline__ void gd_pair_atoms_load(
     for (int i = 0; i < GD_ROW; ++i) cur[i] = rw[i];
     if (j + 1 < LC) {
       const GD_AS_CONST float *__restrict__ prm =
-          (const GD_AS_CONST float *)(ptab + (toff[j + 1 < LC ? j + 1 : j] + poff));
+          (const GD_AS_CONST float *)(ptab + (trow[j + 1 < LC ? j + 1 : j] * GD_ROW_BLOCK + poff));
 #pragma unroll
       for (int i = 0; i < GD_ROW; ++i) rw[i] = prm[i];
     }
@@ -257,9 +259,8 @@ template <int LC, int CLS>
 __device__ __forceinline__ void gd_pair_class(
     const GD_AS_CONST GdJob &J, const int lo, const int hi, const int b,
     const GD_AS_GLOBAL float4 *__restrict__ pc, const GdPairConst &k, const float pmf_a,
-    const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC], const unsigned (&toff)[LC],
-    const unsigned (&trow)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
-    float &epsp, float &ehdb) {
+    const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC], const unsigned (&trow)[LC],
+    float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf, float &epsp, float &ehdb) {
   const int P = J.P, nP = J.nP;
   const int p0 = lo + b;
   if (p0 >= hi) return;
@@ -287,14 +288,14 @@ __device__ __forceinline__ void gd_pair_class(
     if constexpr (kRows) {
       if (ty != ty_cur) {
         ty_cur = ty;
-        gd_load_rows<LC>(ptab, toff, poff, *reinterpret_cast<float(*)[LC][GD_ROW]>(&rw));
+        gd_load_rows<LC>(ptab, trow, poff, *reinterpret_cast<float(*)[LC][GD_ROW]>(&rw));
       }
       gd_pair_atoms_rows<LC, CLS>(k, *reinterpret_cast<float(*)[LC][GD_ROW]>(&rw), pspv, P, q, c,
                                   pa.x, pa.y, pmf_a, lx, ly, lz, trow, ele, hpc, evdw, epmf, epsp,
                                   ehdb);
     } else {
-      gd_pair_atoms_load<LC, CLS>(k, ptab, pspv, P, q, poff, c, pa.x, pa.y, pmf_a, lx, ly, lz, toff,
-                                  trow, ele, hpc, evdw, epmf, epsp, ehdb);
+      gd_pair_atoms_load<LC, CLS>(k, ptab, pspv, P, q, poff, c, pa.x, pa.y, pmf_a, lx, ly, lz, trow,
+                                  ele, hpc, evdw, epmf, epsp, ehdb);
     }
   };
   // Two points per iteration, each prefetching the next point into the other
@@ -319,18 +320,18 @@ template <int LC>
 __device__ __forceinline__ void gd_pair_points(
     const GD_AS_CONST GdJob &J, const int b, const GD_AS_GLOBAL float4 *__restrict__ pc,
     const GdPairConst &k, const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC],
-    const unsigned (&toff)[LC], const unsigned (&trow)[LC], float (&ele)[LC], float (&hpc)[LC],
-    float &evdw, float &epmf, float &epsp, float &ehdb) {
+    const unsigned (&trow)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
+    float &epsp, float &ehdb) {
   // log2(e)/2 held in a VGPR: a VALU op reads at most one SGPR or literal
   float pmf_a;
   asm("v_mov_b32 %0, 0x3f38aa3b" : "=v"(pmf_a));  // log2(e) / 2
-  gd_pair_class<LC, 0>(J, J.pcls[0], J.pcls[1], b, pc, k, pmf_a, lx, ly, lz, toff, trow, ele, hpc,
+  gd_pair_class<LC, 0>(J, J.pcls[0], J.pcls[1], b, pc, k, pmf_a, lx, ly, lz, trow, ele, hpc,
                        evdw, epmf, epsp, ehdb);
-  gd_pair_class<LC, 1>(J, J.pcls[1], J.pcls[2], b, pc, k, pmf_a, lx, ly, lz, toff, trow, ele, hpc,
+  gd_pair_class<LC, 1>(J, J.pcls[1], J.pcls[2], b, pc, k, pmf_a, lx, ly, lz, trow, ele, hpc,
                        evdw, epmf, epsp, ehdb);
-  gd_pair_class<LC, 2>(J, J.pcls[2], J.pcls[3], b, pc, k, pmf_a, lx, ly, lz, toff, trow, ele, hpc,
+  gd_pair_class<LC, 2>(J, J.pcls[2], J.pcls[3], b, pc, k, pmf_a, lx, ly, lz, trow, ele, hpc,
                        evdw, epmf, epsp, ehdb);
-  gd_pair_class<LC, 3>(J, J.pcls[3], J.pcls[4], b, pc, k, pmf_a, lx, ly, lz, toff, trow, ele, hpc,
+  gd_pair_class<LC, 3>(J, J.pcls[3], J.pcls[4], b, pc, k, pmf_a, lx, ly, lz, trow, ele, hpc,
                        evdw, epmf, epsp, ehdb);
 }
 
@@ -349,7 +350,7 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
 
   // ligand chunk: coordinates of this lane's replica, per-atom constants (uniform)
   float lx[LC], ly[LC], lz[LC], ele[LC], hpc[LC];
-  unsigned toff[LC], trow[LC];  // row block of each atom: byte offset, index
+  unsigned trow[LC];  // row block of each atom
 #pragma unroll
   for (int j = 0; j < LC; ++j) {
     const int l = l0 + min(j, nl - 1);
@@ -357,7 +358,6 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
     ly[j] = coords[(size_t)(3 * l + 1) * Rp + r];
     lz[j] = coords[(size_t)(3 * l + 2) * Rp + r];
     trow[j] = (unsigned)atoms[l0 + j].tidx;
-    toff[j] = trow[j] * (GD_PTAB_PT * GD_PTAB_W * 4);
     ele[j] = 0.0f;
     hpc[j] = 0.0f;
   }
@@ -372,7 +372,7 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   k.inv_el1 = J.inv_el1;
   const float inv_el1 = k.inv_el1;
   float evdw = 0.0f, epmf = 0.0f, epsp = 0.0f, ehdb = 0.0f;
-  gd_pair_points<LC>(J, b, pc, k, lx, ly, lz, toff, trow, ele, hpc, evdw, epmf, epsp, ehdb);
+  gd_pair_points<LC>(J, b, pc, k, lx, ly, lz, trow, ele, hpc, evdw, epmf, epsp, ehdb);
   ehdb *= GD_HDB_RC;
 
   // KDE: this tile's share of every chunk atom's anchor bin, a contiguous range
@@ -455,7 +455,7 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
 
 // One work group of GD_GROUP waves per item (grid = items).
 template <int LC>
-__global__ __launch_bounds__(64 * GD_GROUP) void gd_pair_kernel(const GdJob *__restrict__ jobs,
+__global__ __launch_bounds__(64 * GD_GROUP) __attribute__((amdgpu_waves_per_eu(LC == 4 ? 8 : 1))) void gd_pair_kernel(const GdJob *__restrict__ jobs,
                                                        const GdItem *__restrict__ items,
                                                        int n_items) {
   __shared__ float red[GD_GROUP][GD_NLIN + LC][64];
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(64 * GD_GROUP) void gd_pair_kernel(const GdJob *__r
 // counter is zeroed by the kernel that runs before each launch (finish or
 // transform kernel); every group exits once the counter passes n_items.
 template <int LC>
-__global__ __launch_bounds__(64 * GD_GROUP) void gd_pair_kernel_wq(const GdJob *__restrict__ jobs,
+__global__ __launch_bounds__(64 * GD_GROUP) __attribute__((amdgpu_waves_per_eu(LC == 4 ? 8 : 1))) void gd_pair_kernel_wq(const GdJob *__restrict__ jobs,
                                                           const GdItem *__restrict__ items,
                                                           int n_items, int *__restrict__ wq) {
   __shared__ float red[GD_GROUP][GD_NLIN + LC][64];
