@@ -251,7 +251,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(jobs and (ligand, protein) replica groups shard across GPUs without communication)" % args.gpus,
+              "(jobs shard across GPUs without communication)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "copy":
