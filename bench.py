@@ -48,12 +48,30 @@ def algorithmic_flops_per_step(job):
     return PRT_FLOPS * prt + KDE_FLOPS * kde + MCS_FLOPS * mcs, prt
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Host threads for the CPU leg: the job's CPU share (OMP_NUM_THREADS when
+    set, at most 16), one per core, threads bound (OMP_PROC_BIND)."""
+    os.environ.setdefault("OMP_PROC_BIND", "true")
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
 def cpu_baseline(job, mc_template, budget_s=15.0):
     """Time the CPU oracle on all replicas for a bounded number of MC steps."""
+    threads = cpu_threads()
     from tests.oracle_ffi import Oracle
     import geauxdock_amd as g
     orc = Oracle()
-    threads = min(16, os.cpu_count() or 1)
     # calibrate on one step, then run a sample of about budget_s seconds
     mc1 = g.make_mc(1, move_mode="gaussian", seed=SEED)
     t0 = time.perf_counter()
@@ -67,7 +85,39 @@ def cpu_baseline(job, mc_template, budget_s=15.0):
     rs = job.n_rep * (steps + 1) / (ms / 1000.0)
     return {"value": rs, "unit": "replica-steps/s", "cores": threads, "kind": "port",
             "sample": "all %d replicas x %d MC steps (+ initial pass) of the same complex, "
-                      "OpenMP over replicas" % (job.n_rep, steps)}
+                      "OpenMP over replicas" % (job.n_rep, steps),
+            "cpu_model": cpu_model()}
+
+
+def plain_c1():
+    """The CPU reference configuration in full: P=1200 points x 5 conformers,
+    L=24 atoms x 5 conformers, 1 temperature (25 replicas), 3000 MC steps,
+    Gaussian moves N(0, 0.2 A) / N(0, 3 deg), seed 1; the CPU oracle (OpenMP over
+    replicas) and the plain GPU path on the same job and draws."""
+    import geauxdock_amd as g
+    threads = cpu_threads()
+    from tests.oracle_ffi import Oracle
+    job = g.synth_complex(1200, 24, 5, 5, 1, floor_temp=0.044, ceiling_temp=0.044, seed=1)
+    mc = g.make_mc(3000, steps_per_dump=3000, move_mode="gaussian", seed=1)
+    dev = g.DeviceJob(job)
+    dev.dock(g.make_mc(100, move_mode="gaussian", seed=1), stats=False)   # warm-up
+    t0 = time.perf_counter()
+    out, st = dev.dock(mc, stats=True)
+    tg = time.perf_counter() - t0
+    orc = Oracle()
+    ref, _, ms = orc.dock(job, mc, threads=threads)
+    rs_gpu = job.n_rep * 3000 / tg
+    rs_cpu = job.n_rep * 3001 / (ms / 1000.0)
+    pa, pb = out.n_accept.mean() / 3000, ref.n_accept.mean() / 3000
+    return {"workload": "c1", "replicas": job.n_rep, "mc_steps": 3000,
+            "gpu_replica_steps_per_s": rs_gpu, "gpu_ms": tg * 1000.0,
+            "cpu_replica_steps_per_s": rs_cpu, "cpu_ms": ms, "cpu_threads": threads,
+            "cpu_model": cpu_model(), "gpu_over_cpu": rs_gpu / rs_cpu,
+            "acceptance_gpu": pa, "acceptance_cpu": pb,
+            "best_energy_mean_gpu": float(out.best_energy.mean()),
+            "best_energy_mean_cpu": float(ref.best_energy.mean()),
+            "note": "25 replicas fill a fraction of either device; the CPU leg is the whole job, "
+                    "MC loop only (initial pass included)"}
 
 
 def _sizes(rng, n, kind):
@@ -101,8 +151,10 @@ def batched(kind, n_jobs, mc_steps, plain_jobs):
                      axes=AXES) for i in range(n_jobs)]
     arr = (C.c_void_p * n_jobs)(*[d.h.value for d in devs])
     mca = (abi.gd_mc_param * n_jobs)(*mcs)
+    outs = [g.host.ReplicaOut(h.n_rep) for h in hosts]
+    oa = (abi.gd_replica_out * n_jobs)(*[o.c for o in outs])
     t0 = time.perf_counter()
-    g.check(g.lib().gd_batch_dock(arr, n_jobs, mca, None, None), "gd_batch_dock")
+    g.check(g.lib().gd_batch_dock(arr, n_jobs, mca, oa, None), "gd_batch_dock")
     tb = time.perf_counter() - t0
     rep_steps = sum(h.n_rep for h in hosts) * mc_steps
     # plain path: the first plain_jobs complexes one at a time, default tiling
@@ -112,10 +164,19 @@ def batched(kind, n_jobs, mc_steps, plain_jobs):
         dp.dock(mcs[i], stats=False)
         dp.close()
     tp = time.perf_counter() - t0
+    # each compared job's best energies against its own plain run (same tiling)
+    same = []
+    for i in range(min(2, n_jobs)):
+        dp = g.DeviceJob(hosts[i])
+        dp.set_tiling(4, min(64, hosts[i].desc.prt_npoint))
+        o, _ = dp.dock(mcs[i], stats=False)
+        dp.close()
+        same.append(bool(np.array_equal(o.best_energy, outs[i].best_energy)))
     return {"workload": kind, "jobs": n_jobs, "mc_steps": mc_steps,
             "batched_jobs_per_s": n_jobs / tb, "batched_replica_steps_per_s": rep_steps / tb,
             "plain_jobs_per_s": plain_jobs / tp,
-            "speedup_jobs_per_s": (n_jobs / tb) / (plain_jobs / tp), "plain_sample_jobs": plain_jobs}
+            "speedup_jobs_per_s": (n_jobs / tb) / (plain_jobs / tp), "plain_sample_jobs": plain_jobs,
+            "best_energy_bitwise_equal_to_plain": same}
 
 
 def serve(rates, n_jobs, step_scale):
@@ -236,8 +297,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100, help="exchange intervals (100 MC steps each)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "copy"], default="c2",
-                    help="c2 = the headline line; c3/c4 batched vs plain; c5 serving (extra lines); "
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "copy"], default="c2",
+                    help="c2 = the headline line; c1 the CPU reference configuration on CPU and GPU; "
+                         "c3/c4 batched vs plain; c5 serving (extra lines); "
                          "copy = a plain 256 MiB device copy, the calibration for HBM counters")
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                     help="summarise rocprofv3 FETCH_SIZE/WRITE_SIZE runs into "
@@ -260,6 +322,9 @@ def main():
     import geauxdock_amd as g
     from geauxdock_amd import abi
     g.check(g.lib().gd_init(0), "gd_init")
+    if args.workload == "c1":
+        print(json.dumps(plain_c1()), flush=True)
+        return
     if args.workload in ("c3", "c4"):
         print(json.dumps(batched(args.workload, args.jobs, args.mc_steps, max(1, args.jobs // 8))),
               flush=True)
@@ -317,7 +382,8 @@ def main():
                                "ligand_conformer": int(out.n_swap_lig.sum()) // 2,
                                "protein_conformer": int(out.n_swap_prt.sum()) // 2},
             "tiling": {"atoms_per_lane": dev.tiling()[0], "tile_points": dev.tiling()[1]},
-            "device": name.value.decode(), "cu": ncu.value,
+            "device": name.value.decode(), "cu": ncu.value, "sclk_mhz": clk.value / 1000.0,
+            "peak_from_device_tflops": ncu.value * 64 * 2 * 2 * clk.value * 1e3 / 1e12,
         },
         "jobs_per_s": 1.0 / wall * (10000.0 / steps),
         "pair_evals_per_s": prt_pairs * (steps + 1) / wall,

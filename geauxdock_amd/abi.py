@@ -86,6 +86,13 @@ class gd_server_config(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("sweep_steps", C.c_int)]
 
 
+class gd_request_files(C.Structure):
+    _fields_ = [
+        ("sdf", C.c_char_p), ("ff", C.c_char_p), ("pdb", C.c_char_p), ("ligand_id", C.c_char_p),
+        ("ene", C.POINTER(gd_enepara)), ("n_tmp", C.c_int), ("floor_temp", _f), ("ceiling_temp", _f),
+    ]
+
+
 class gd_job_result(C.Structure):
     _fields_ = [
         ("ticket", C.c_int64), ("status", C.c_int), ("t_submit", C.c_double),
@@ -126,6 +133,9 @@ PROTOTYPES = {
     "gd_server_cancel": (C.c_int, [_vp, C.c_int64]),
     "gd_server_results": (C.c_int, [_vp, C.POINTER(gd_job_result), C.c_int, _ip]),
     "gd_server_clock": (C.c_double, [_vp]),
+    "gd_server_submit_files": (C.c_int, [_vp, C.POINTER(gd_request_files), C.POINTER(gd_mc_param),
+                                         C.POINTER(C.c_int64)]),
+    "gd_server_take": (C.c_int, [_vp, C.c_int64, _ip, C.POINTER(gd_replica_out)]),
     "gd_load_enepara": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                   C.POINTER(gd_enepara)]),
     "gd_load_complex": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,

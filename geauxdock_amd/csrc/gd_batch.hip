@@ -14,10 +14,13 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "gd_internal.h"
@@ -476,10 +479,47 @@ struct ServJob {
   gd_replica_out *out;
   size_t foot;
   gd_job *job = nullptr;
+  gd_host_job *host = nullptr;  // file requests: the parsed complex behind desc
   int done_steps = 0;
   int step0 = 0;
   double t_submit = 0, t_admit = 0;
   bool init_done = false;
+  ~ServJob() {
+    if (host) gd_host_job_free(host);
+  }
+};
+
+// A request given by its files, waiting for the loader thread.
+struct FileReq {
+  int64_t ticket;
+  std::string sdf, ff, pdb, id;
+  const gd_enepara *ene;
+  int n_tmp;
+  float floor_temp, ceiling_temp;
+  gd_mc_param mc;
+  double t_submit;
+};
+
+// Server-owned final replica state of a file request (until gd_server_take).
+struct HeldOut {
+  int n_rep = 0;
+  bool done = false;
+  std::vector<float> mm, en, best;
+  std::vector<int> tmp, nacc, nexc, acc, nsl, nsp;
+  gd_replica_out view{};
+  explicit HeldOut(int n)
+      : n_rep(n), mm((size_t)n * 6), en((size_t)n * 10), best(n), tmp(n), nacc(n), nexc(n), acc(n),
+        nsl(n), nsp(n) {
+    view.movematrix = mm.data();
+    view.energy = en.data();
+    view.best_energy = best.data();
+    view.idx_tmp = tmp.data();
+    view.n_accept = nacc.data();
+    view.n_exchange = nexc.data();
+    view.is_accept = acc.data();
+    view.n_swap_lig = nsl.data();
+    view.n_swap_prt = nsp.data();
+  }
 };
 
 struct gd_server {
@@ -495,7 +535,14 @@ struct gd_server {
   bool dirty = true;
   int lc = kSharedLc;
   std::chrono::steady_clock::time_point t0;
-  std::mutex mu;
+  std::mutex mu;                 // everything above, and held
+  std::map<int64_t, HeldOut *> held;
+  // host input pipeline: file requests parsed by a loader thread
+  std::mutex fmu;                // files, stop
+  std::condition_variable fcv;
+  std::deque<FileReq *> files;
+  bool stop = false;
+  std::thread loader;
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -504,6 +551,54 @@ struct gd_server {
 namespace gdr {
 size_t footprint(const gd_job_desc *d);
 }
+
+// Loader thread: parses file requests outside every lock but its queue's, so
+// parsing overlaps the device work of gd_server_step (which holds mu), then
+// queues the parsed job for admission.
+static void server_loader(gd_server *s) {
+  for (;;) {
+    FileReq *r;
+    {
+      std::unique_lock<std::mutex> lk(s->fmu);
+      s->fcv.wait(lk, [&] { return s->stop || !s->files.empty(); });
+      if (s->stop) return;
+      r = s->files.front();
+      s->files.pop_front();
+    }
+    gd_host_job *h = nullptr;
+    gd_job_desc d;
+    std::memset(&d, 0, sizeof d);
+    size_t foot = 0;
+    gd_status st = gd_load_complex(r->sdf.c_str(), r->ff.c_str(), r->pdb.c_str(), r->id.c_str(),
+                                   r->ene, r->n_tmp, r->floor_temp, r->ceiling_temp, &h);
+    if (st == GD_OK) st = gd_host_job_desc(h, &d);
+    if (st == GD_OK) {
+      foot = footprint(&d);
+      if (foot == 0) st = GD_ERR_ARG;
+      else if (foot > s->cfg.hbm_budget) st = GD_ERR_BUSY;  // can never be admitted
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (st != GD_OK) {
+      if (h) gd_host_job_free(h);
+      s->results.push_back({r->ticket, st, r->t_submit, 0.0, s->now(), 0.0});
+      delete r;
+      continue;
+    }
+    HeldOut *o = new HeldOut(d.n_lig * d.n_prt * d.n_tmp);
+    s->held[r->ticket] = o;
+    ServJob *j = new ServJob();
+    j->ticket = r->ticket;
+    j->desc = d;
+    j->mc = r->mc;
+    j->out = &o->view;
+    j->foot = foot;
+    j->host = h;
+    j->t_submit = r->t_submit;
+    s->pending.push_back(j);
+    delete r;
+  }
+}
+
 
 extern "C" gd_status gd_server_create(const gd_server_config *cfg, gd_server **out) {
   if (!cfg || !out || cfg->sweep_steps <= 0 || cfg->max_active <= 0) return GD_ERR_ARG;
@@ -523,6 +618,14 @@ extern "C" gd_status gd_server_create(const gd_server_config *cfg, gd_server **o
 
 extern "C" gd_status gd_server_destroy(gd_server *s) {
   if (!s) return GD_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->fmu);
+    s->stop = true;
+  }
+  s->fcv.notify_all();
+  if (s->loader.joinable()) s->loader.join();
+  for (FileReq *r : s->files) delete r;
+  for (auto &kv : s->held) delete kv.second;
   (void)hipStreamSynchronize(s->stream);
   for (ServJob *j : s->active) {
     gd_job_destroy(j->job);
@@ -555,9 +658,92 @@ extern "C" gd_status gd_server_submit(gd_server *s, const gd_job_desc *desc, con
   return GD_OK;
 }
 
-extern "C" gd_status gd_server_cancel(gd_server *s, int64_t ticket) {
+extern "C" gd_status gd_server_submit_files(gd_server *s, const gd_request_files *req,
+                                            const gd_mc_param *mc, int64_t *ticket) {
+  if (!s || !req || !mc || mc->steps_total < 0 || !req->sdf || !req->ff || !req->pdb ||
+      !req->ligand_id || !req->ene || req->n_tmp < 1)
+    return GD_ERR_ARG;
+  FileReq *r = new FileReq();
+  r->sdf = req->sdf;
+  r->ff = req->ff;
+  r->pdb = req->pdb;
+  r->id = req->ligand_id;
+  r->ene = req->ene;
+  r->n_tmp = req->n_tmp;
+  r->floor_temp = req->floor_temp;
+  r->ceiling_temp = req->ceiling_temp;
+  r->mc = *mc;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->ticket = s->next_ticket++;
+    r->t_submit = s->now();
+  }
+  const int64_t t = r->ticket;
+  {
+    std::lock_guard<std::mutex> lk(s->fmu);
+    s->files.push_back(r);
+    if (!s->loader.joinable()) s->loader = std::thread(server_loader, s);
+  }
+  s->fcv.notify_one();
+  if (ticket) *ticket = t;
+  return GD_OK;
+}
+
+extern "C" gd_status gd_server_take(gd_server *s, int64_t ticket, int *n_rep, gd_replica_out *out) {
   if (!s) return GD_ERR_ARG;
   std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->held.find(ticket);
+  if (it == s->held.end()) return GD_ERR_ARG;
+  HeldOut *o = it->second;
+  if (!o->done) return GD_ERR_STATE;
+  if (n_rep) *n_rep = o->n_rep;
+  if (!out) return GD_OK;
+  const size_t n = (size_t)o->n_rep;
+  auto cp = [](void *dst, const void *src, size_t bytes) {
+    if (dst) std::memcpy(dst, src, bytes);
+  };
+  cp(out->movematrix, o->mm.data(), n * 6 * sizeof(float));
+  cp(out->energy, o->en.data(), n * 10 * sizeof(float));
+  cp(out->best_energy, o->best.data(), n * sizeof(float));
+  cp(out->idx_tmp, o->tmp.data(), n * sizeof(int));
+  cp(out->n_accept, o->nacc.data(), n * sizeof(int));
+  cp(out->n_exchange, o->nexc.data(), n * sizeof(int));
+  cp(out->is_accept, o->acc.data(), n * sizeof(int));
+  cp(out->n_swap_lig, o->nsl.data(), n * sizeof(int));
+  cp(out->n_swap_prt, o->nsp.data(), n * sizeof(int));
+  delete o;
+  s->held.erase(it);
+  return GD_OK;
+}
+
+extern "C" gd_status gd_server_cancel(gd_server *s, int64_t ticket) {
+  if (!s) return GD_ERR_ARG;
+  {  // a file request not parsed yet
+    FileReq *r = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(s->fmu);
+      for (auto it = s->files.begin(); it != s->files.end(); ++it)
+        if ((*it)->ticket == ticket) {
+          r = *it;
+          s->files.erase(it);
+          break;
+        }
+    }
+    if (r) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->results.push_back({ticket, GD_ERR_STATE, r->t_submit, 0.0, s->now(), 0.0});
+      delete r;
+      return GD_OK;
+    }
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  {
+    auto h = s->held.find(ticket);
+    if (h != s->held.end() && !h->second->done) {  // cancelled: nothing to take
+      delete h->second;
+      s->held.erase(h);
+    }
+  }
   for (auto it = s->pending.begin(); it != s->pending.end(); ++it)
     if ((*it)->ticket == ticket) {
       s->results.push_back({ticket, GD_ERR_STATE, (*it)->t_submit, 0.0, s->now(), 0.0});
@@ -583,9 +769,14 @@ extern "C" gd_status gd_server_cancel(gd_server *s, int64_t ticket) {
 
 extern "C" gd_status gd_server_usage(gd_server *s, size_t *used, int *n_pending) {
   if (!s) return GD_ERR_ARG;
+  int nf;
+  {
+    std::lock_guard<std::mutex> lk(s->fmu);
+    nf = (int)s->files.size();
+  }
   std::lock_guard<std::mutex> lk(s->mu);
   if (used) *used = s->used;
-  if (n_pending) *n_pending = (int)s->pending.size();
+  if (n_pending) *n_pending = (int)s->pending.size() + nf;
   return GD_OK;
 }
 
@@ -611,9 +802,17 @@ extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
          s->used + s->pending.front()->foot <= s->cfg.hbm_budget) {
     ServJob *j = s->pending.front();
     s->pending.pop_front();
+    auto drop_held = [&](int64_t t) {  // a failed file request leaves nothing to take
+      auto h = s->held.find(t);
+      if (h != s->held.end()) {
+        delete h->second;
+        s->held.erase(h);
+      }
+    };
     st = gd_job_create(&j->desc, &j->job);
     if (st != GD_OK) {
       s->results.push_back({j->ticket, st, j->t_submit, s->now(), s->now(), 0.0});
+      drop_held(j->ticket);
       delete j;
       continue;
     }
@@ -622,6 +821,7 @@ extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
       if ((st = apply_tiling(j->job, s->lc, shared_ptile(j->job->h.P))) != GD_OK ||
           (st = sync_job_struct(j->job)) != GD_OK) {
         s->results.push_back({j->ticket, st, j->t_submit, s->now(), s->now(), 0.0});
+        drop_held(j->ticket);
         gd_job_destroy(j->job);
         delete j;
         continue;
@@ -691,6 +891,8 @@ extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
     ServJob *j = *it;
     if (j->done_steps >= j->mc.steps_total) {
       gd_status rs = read_out(j->job, j->out);
+      auto h = s->held.find(j->ticket);
+      if (h != s->held.end()) h->second->done = true;
       s->results.push_back({j->ticket, rs, j->t_submit, j->t_admit, s->now(),
                             (double)j->job->h.R * j->mc.steps_total});
       s->used -= j->foot;

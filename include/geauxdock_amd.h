@@ -249,8 +249,32 @@ gd_status gd_server_cancel(gd_server *s, int64_t ticket);
 /* Pop up to `cap` finished results. */
 gd_status gd_server_results(gd_server *s, gd_job_result *res, int cap, int *n);
 double gd_server_clock(gd_server *s);
-/* Bytes of the budget in use and requests still waiting for admission. */
+/* Bytes of the budget in use and requests still waiting for admission
+ * (file requests still being parsed included). */
 gd_status gd_server_usage(gd_server *s, size_t *bytes_used, int *n_pending);
+
+/* A docking request given by its files: the inputs of gd_load_complex. */
+typedef struct {
+  const char *sdf, *ff, *pdb, *ligand_id;   /* copied at submission */
+  const gd_enepara *ene;                    /* must stay valid until the request is parsed */
+  int n_tmp;
+  float floor_temp, ceiling_temp;
+} gd_request_files;
+/* Queue a request by its files.  The server's loader thread parses them (the
+ * host input pipeline of server1.C:54-217: ligand, restraint and protein
+ * loaders with their optimisation passes) while the device runs the active
+ * jobs; the parsed job then waits for admission like a gd_server_submit
+ * request.  The server keeps its final replica state until gd_server_take.
+ * A request that fails to parse, or can never fit the budget, comes back as a
+ * result with that status. */
+gd_status gd_server_submit_files(gd_server *s, const gd_request_files *req, const gd_mc_param *mc,
+                                 int64_t *ticket);
+/* Final replica state of a finished file request.  With out == NULL, *n_rep
+ * receives its replica count and the state stays; with out (arrays of *n_rep
+ * entries; any pointer may be NULL) it is copied out and released.
+ * GD_ERR_STATE while the job has not finished, GD_ERR_ARG for an unknown
+ * ticket. */
+gd_status gd_server_take(gd_server *s, int64_t ticket, int *n_rep, gd_replica_out *out);
 
 /* ---- loaders (reference frontend, re-implemented) ----------------------- */
 typedef struct gd_host_job gd_host_job;   /* host-side complex built by the loaders */

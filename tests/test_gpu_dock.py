@@ -264,6 +264,60 @@ def test_server_alone_vs_mix(ene):
     assert alone.n_swap_lig.sum() > 0 and alone.n_swap_prt.sum() > 0
 
 
+def test_server_file_requests(ene, golden_job):
+    """Host input pipeline: a request given by its files is parsed by the
+    server's loader thread and docks to the same bits as the same complex
+    submitted as a parsed descriptor; a request whose files are missing comes
+    back as a result with the loader's status and leaves nothing to take."""
+    import ctypes as C
+    import os
+    import time
+    from tests.conftest import GOLDEN_DATA as d
+    mc = g.make_mc(60, steps_per_exchange=20, move_mode="gaussian", seed=3, job_id=5,
+                   axes=("temp", "lig", "prt"))
+    cfg = abi.gd_server_config(hbm_budget=16 << 30, max_active=4, sweep_steps=20)
+    s = C.c_void_p()
+    g.check(g.lib().gd_server_create(C.byref(cfg), C.byref(s)), "gd_server_create")
+    try:
+        def files(sdf):
+            return abi.gd_request_files(sdf.encode(), os.path.join(d, "1a07C1-0.8.ff").encode(),
+                                        os.path.join(d, "1a07C-1.pdb").encode(), b"1a07C1",
+                                        C.pointer(ene), 1, 0.044, 0.036)
+        req, bad = files(os.path.join(d, "1a07C1.sdf")), files(os.path.join(d, "missing.sdf"))
+        tf, tb, td = C.c_int64(), C.c_int64(), C.c_int64()
+        g.check(g.lib().gd_server_submit_files(s, C.byref(req), C.byref(mc), C.byref(tf)), "submit_files")
+        g.check(g.lib().gd_server_submit_files(s, C.byref(bad), C.byref(mc), C.byref(tb)), "submit_files")
+        o_desc = g.host.ReplicaOut(golden_job.n_rep)
+        g.check(g.lib().gd_server_submit(s, C.byref(golden_job.desc), C.byref(mc), C.byref(o_desc.c),
+                                         C.byref(td)), "submit")
+        status = {}
+        for _ in range(400):
+            na, nd = C.c_int(), C.c_int()
+            g.check(g.lib().gd_server_step(s, C.byref(na), C.byref(nd)), "step")
+            res = (abi.gd_job_result * 8)()
+            n = C.c_int()
+            g.lib().gd_server_results(s, res, 8, C.byref(n))
+            for i in range(n.value):
+                status[res[i].ticket] = res[i].status
+            if len(status) == 3:
+                break
+            if na.value == 0:
+                time.sleep(0.01)  # the loader thread is still parsing
+        assert status.get(tf.value) == 0 and status.get(td.value) == 0, status
+        assert status.get(tb.value, 0) != 0, status
+        n_rep = C.c_int()
+        assert g.lib().gd_server_take(s, tf, C.byref(n_rep), None) == abi.GD_OK
+        assert n_rep.value == golden_job.n_rep
+        o_file = g.host.ReplicaOut(n_rep.value)
+        assert g.lib().gd_server_take(s, tf, None, C.byref(o_file.c)) == abi.GD_OK
+        for f in ("energy", "movematrix", "idx_tmp", "n_accept", "n_swap_lig", "best_energy"):
+            assert np.array_equal(getattr(o_file, f), getattr(o_desc, f)), f
+        assert g.lib().gd_server_take(s, tf, C.byref(n_rep), None) == abi.GD_ERR_ARG  # released
+        assert g.lib().gd_server_take(s, tb, C.byref(n_rep), None) == abi.GD_ERR_ARG
+    finally:
+        g.lib().gd_server_destroy(s)
+
+
 def test_server_admission_budget(ene):
     """A request larger than the whole budget is refused; requests that fit
     only one at a time run one after another."""
