@@ -85,7 +85,8 @@ def cpu_baseline(job, mc_template, budget_s=15.0):
     rs = job.n_rep * (steps + 1) / (ms / 1000.0)
     return {"value": rs, "unit": "replica-steps/s", "cores": threads, "kind": "port",
             "sample": "all %d replicas x %d MC steps (+ initial pass) of the same complex, "
-                      "OpenMP over replicas" % (job.n_rep, steps),
+                      "OpenMP over replicas; the reference backend's fixed capacities (2048 protein "
+                      "points) cannot hold this complex, so its restatement is timed" % (job.n_rep, steps),
             "cpu_model": cpu_model()}
 
 
@@ -109,7 +110,23 @@ def plain_c1():
     rs_gpu = job.n_rep * 3000 / tg
     rs_cpu = job.n_rep * 3001 / (ms / 1000.0)
     pa, pb = out.n_accept.mean() / 3000, ref.n_accept.mean() / 3000
-    return {"workload": "c1", "replicas": job.n_rep, "mc_steps": 3000,
+    # the reference CPU backend itself (compiled from its sources into
+    # oracle/_ref, where that was built), same job, its own moves and RNG
+    ref_leg = None
+    lib = os.path.join(ROOT, "oracle", "_ref", "libgdref.so")
+    if os.path.exists(lib):
+        from geauxdock_amd import abi
+        R = C.CDLL(lib)
+        R.ref_dock.argtypes = [C.POINTER(abi.gd_job_desc), C.c_int, C.c_int, C.POINTER(C.c_float),
+                               C.POINTER(C.c_double), C.POINTER(C.c_float)]
+        rms = C.c_double()
+        scale = (C.c_float * 6)(*[float(x) for x in mc.move_scale])
+        if R.ref_dock(C.byref(job.desc), 3000, 200, scale, C.byref(rms), None) == 0:
+            ref_leg = {"replica_steps_per_s": job.n_rep * 3001 / (rms.value / 1000.0), "ms": rms.value,
+                       "threads": threads, "kind": "reference",
+                       "note": "backend_cpu_mic Dock compiled from the reference sources, uniform moves "
+                               "of the same scale, its own RNG"}
+    return {"workload": "c1", "replicas": job.n_rep, "mc_steps": 3000, "cpu_reference": ref_leg,
             "gpu_replica_steps_per_s": rs_gpu, "gpu_ms": tg * 1000.0,
             "cpu_replica_steps_per_s": rs_cpu, "cpu_ms": ms, "cpu_threads": threads,
             "cpu_model": cpu_model(), "gpu_over_cpu": rs_gpu / rs_cpu,

@@ -11,6 +11,10 @@
 // move amount is 0 and the accept draw is 0, so the energy of the stored pose
 // lands in replica[r].energy.  A temperature of -minus_beta = 1e-30 keeps the
 // forced accept from underflowing.
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -21,20 +25,10 @@
 
 #include "../../include/geauxdock_amd.h"
 
-extern "C" int ref_energy(const gd_job_desc *d, int n, const int *conf, const float *pose,
-                          float *terms) {
+// The job's tables in the reference's own Complex (everything but replicas,
+// temperatures and MC parameters).
+static void fill_complex(const gd_job_desc *d, Complex *c) {
   const int L = d->lig_natom, P = d->prt_npoint, K = d->kde_npoint, M = d->mcs_nrow;
-  if (L > MAXLIG || P > MAXPRO || K > MAXKDE || M > MAX_MCS_ROW || d->n_lig > MAX_CONF_LIG ||
-      d->n_prt > MAX_CONF_PRT || n > MAX_REP || d->psp_ncol > MAXPRO)
-    return 3;  // outside the reference's fixed capacities
-  // the reference types are 64-byte aligned (geauxdock.h:29); -O3 vector
-  // code relies on it, so allocate accordingly and zero-fill
-  const size_t cb = (sizeof(Complex) + 63) / 64 * 64, rb = ((size_t)n * sizeof(Record) + 63) / 64 * 64;
-  Complex *c = (Complex *)aligned_alloc(64, cb);
-  Record *rec = (Record *)aligned_alloc(64, rb);
-  if (!c || !rec) return 6;
-  memset((void *)c, 0, cb);
-  memset((void *)rec, 0, rb);
   for (int i = 0; i < d->n_lig; ++i) {
     Ligand *g = &c->lig[i];
     g->lig_natom = L;
@@ -109,6 +103,38 @@ extern "C" int ref_energy(const gd_job_desc *d, int n, const int *conf, const fl
   memcpy(ep->a_para, e->a_para, sizeof ep->a_para);
   memcpy(ep->b_para, e->b_para, sizeof ep->b_para);
   memcpy(ep->w, e->w, sizeof ep->w);
+  c->size.n_lig = d->n_lig, c->size.n_prt = d->n_prt;
+  c->size.lig_natom = L, c->size.prt_npoint = P, c->size.kde_npoint = K, c->size.mcs_nrow = M;
+  std::strcpy(c->lig_file, "job");
+  std::strcpy(c->prt_file, "job");
+}
+
+static bool fits(const gd_job_desc *d, int n) {
+  return d->lig_natom <= MAXLIG && d->prt_npoint <= MAXPRO && d->kde_npoint <= MAXKDE &&
+         d->mcs_nrow <= MAX_MCS_ROW && d->n_lig <= MAX_CONF_LIG && d->n_prt <= MAX_CONF_PRT &&
+         n <= MAX_REP && d->psp_ncol <= MAXPRO && d->n_tmp <= MAX_TMP;
+}
+
+// The reference types are 64-byte aligned (geauxdock.h:29); -O3 vector code
+// relies on it, so allocate accordingly and zero-fill.
+static void *alloc64(size_t bytes) {
+  const size_t b = (bytes + 63) / 64 * 64;
+  void *p = aligned_alloc(64, b);
+  if (p) memset(p, 0, b);
+  return p;
+}
+
+extern "C" int ref_energy(const gd_job_desc *d, int n, const int *conf, const float *pose,
+                          float *terms) {
+  if (!fits(d, n)) return 3;  // outside the reference's fixed capacities
+  Complex *c = (Complex *)alloc64(sizeof(Complex));
+  Record *rec = (Record *)alloc64((size_t)n * sizeof(Record));
+  if (!c || !rec) {
+    free(c);
+    free(rec);
+    return 6;
+  }
+  fill_complex(d, c);
   c->temp[0].minus_beta = -1e-30f;
   for (int r = 0; r < n; ++r) {
     ReplicaMC *rep = &c->replica[r];
@@ -118,17 +144,69 @@ extern "C" int ref_energy(const gd_job_desc *d, int n, const int *conf, const fl
     rep->idx_tmp = 0;
     for (int k = 0; k < 6; ++k) rep->movematrix[k] = pose[6 * r + k];
   }
-  c->size.n_lig = d->n_lig, c->size.n_prt = d->n_prt, c->size.n_tmp = 1, c->size.n_rep = n;
-  c->size.lig_natom = L, c->size.prt_npoint = P, c->size.kde_npoint = K, c->size.mcs_nrow = M;
+  c->size.n_tmp = 1, c->size.n_rep = n;
   c->mcpara.steps_total = 0;
   c->mcpara.steps_per_dump = 1;
   c->rep_begin = 0;
   c->rep_end = n - 1;
-  std::strcpy(c->lig_file, "job");
-  std::strcpy(c->prt_file, "job");
   Dock(c, rec);
   for (int r = 0; r < n; ++r)
     for (int k = 0; k < MAXWEI; ++k) terms[(size_t)r * MAXWEI + k] = c->replica[r].energy[k];
+  free(rec);
+  free(c);
+  return 0;
+}
+
+// The reference CPU backend's Monte Carlo on a whole job, timed: every replica
+// in SetReplica order (util.C:370-406), the job's temperature ladder, the
+// reference's own moves (move_scale * U[0,1)) and time-seeded RNG, steps_total
+// steps in dumps of steps_per_dump (<= STEPS_PER_DUMP).  *ms receives the wall
+// time of Dock (initial pass included); its progress lines go to /dev/null.
+extern "C" int ref_dock(const gd_job_desc *d, int steps_total, int steps_per_dump,
+                        const float *move_scale, double *ms, float *final_total) {
+  const int nl = d->n_lig, np = d->n_prt, nt = d->n_tmp, n = nl * np * nt;
+  if (!fits(d, n) || steps_per_dump < 1 || steps_per_dump > STEPS_PER_DUMP) return 3;
+  Complex *c = (Complex *)alloc64(sizeof(Complex));
+  Record *rec = (Record *)alloc64((size_t)n * sizeof(Record));
+  if (!c || !rec) {
+    free(c);
+    free(rec);
+    return 6;
+  }
+  fill_complex(d, c);
+  for (int t = 0; t < nt; ++t) {
+    c->temp[t].minus_beta = d->minus_beta[t];
+    c->temp[t].t = -1.0f / d->minus_beta[t];
+    c->temp[t].order = t;
+  }
+  for (int p = 0; p < np; ++p)
+    for (int t = 0; t < nt; ++t)
+      for (int l = 0; l < nl; ++l) {
+        ReplicaMC *rep = &c->replica[nt * nl * p + nl * t + l];
+        rep->idx_rep = nt * nl * p + nl * t + l;
+        rep->idx_prt = p, rep->idx_tmp = t, rep->idx_lig = l;
+      }
+  c->size.n_tmp = nt, c->size.n_rep = n;
+  c->mcpara.steps_total = steps_total;
+  c->mcpara.steps_per_dump = steps_per_dump;
+  for (int k = 0; k < 6; ++k) c->mcpara.move_scale[k] = move_scale[k];
+  c->rep_begin = 0;
+  c->rep_end = n - 1;
+  fflush(stdout);
+  const int saved = dup(1), devnull = open("/dev/null", O_WRONLY);
+  if (saved >= 0 && devnull >= 0) dup2(devnull, 1);
+  const auto t0 = std::chrono::steady_clock::now();
+  Dock(c, rec);
+  const auto t1 = std::chrono::steady_clock::now();
+  fflush(stdout);
+  if (saved >= 0) {
+    dup2(saved, 1);
+    close(saved);
+  }
+  if (devnull >= 0) close(devnull);
+  *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  if (final_total)
+    for (int r = 0; r < n; ++r) final_total[r] = c->replica[r].energy[MAXWEI - 1];
   free(rec);
   free(c);
   return 0;
