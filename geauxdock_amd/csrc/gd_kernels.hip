@@ -622,22 +622,56 @@ __global__ __launch_bounds__(256) void gd_reduce_kernel(const GdJob *__restrict_
   float acc[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c) acc[c] = 0.0f;
-  {
-    const int q0 = (int)(((long long)nS * wi) / W), q1 = (int)(((long long)nS * (wi + 1)) / W);
+  const int q0 = (int)(((long long)nS * wi) / W), q1 = (int)(((long long)nS * (wi + 1)) / W);
+  const int nG = J.nG, nL = J.nL, L = J.L, M = J.M;
+  if (q1 - q0 <= 4 && nG <= 16 && nL <= 16 && L <= W && M <= W) {
+    // Common shape (at most 4 linear slots, one atom and one MCS row per wave):
+    // every load of the wave is issued before the first sum, one wait in all.
+    // Indices are clamped into range and the extra values masked to zero.
+    const int ql = max(q1 - 1, q0), l = min(wi, L - 1), m = min(wi, max(M - 1, 0));
+    float vl[GD_NLIN][4], vh[16], vm[16];
+#pragma unroll
+    for (int c = 0; c < GD_NLIN; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) vl[c][i] = part_lin[((size_t)c * nS + min(q0 + i, ql)) * Rp + r];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) vh[i] = part_hpc[((size_t)l * nG + min(i, nG - 1)) * Rp + r];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) vm[i] = part_mcs[((size_t)m * nL + min(i, nL - 1)) * Rp + r];
+#pragma unroll
+    for (int c = 0; c < GD_NLIN; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[c] += q0 + i < q1 ? vl[c][i] : 0.0f;
+    float h = 0.0f, d = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      h += i < nG ? vh[i] : 0.0f;
+      d += i < nL ? vm[i] : 0.0f;
+    }
+    if (wi < L) {  // hpc per atom (energy_prt_v1.C:132-133)
+      const GdAtom at = atoms[l];
+      const float t = (h - at.hpl0) / at.hpl1;
+      acc[6] += 0.5f * t * t - at.hpl2;
+    }
+    if (wi < M) {  // MCS rows (energy_mcs_v1.C:5-29)
+      const float2 row = mcsrow[m];
+      acc[7] += row.x * sqrtf(d / row.y);
+    }
+  } else {
 #pragma unroll
     for (int c = 0; c < GD_NLIN; ++c)
       acc[c] = gd_sum_strided(part_lin + ((size_t)c * nS + q0) * Rp + r, q1 - q0, (size_t)Rp);
-  }
-  for (int l = wi; l < J.L; l += W) {  // hpc per atom (energy_prt_v1.C:132-133)
-    const float h = gd_sum_strided(part_hpc + (size_t)l * J.nG * Rp + r, J.nG, (size_t)Rp);
-    const GdAtom at = atoms[l];
-    const float t = (h - at.hpl0) / at.hpl1;
-    acc[6] += 0.5f * t * t - at.hpl2;
-  }
-  for (int m = wi; m < J.M; m += W) {  // MCS rows (energy_mcs_v1.C:5-29)
-    const float d = gd_sum_strided(part_mcs + (size_t)m * J.nL * Rp + r, J.nL, (size_t)Rp);
-    const float2 row = mcsrow[m];
-    acc[7] += row.x * sqrtf(d / row.y);
+    for (int l = wi; l < L; l += W) {  // hpc per atom (energy_prt_v1.C:132-133)
+      const float h = gd_sum_strided(part_hpc + (size_t)l * nG * Rp + r, nG, (size_t)Rp);
+      const GdAtom at = atoms[l];
+      const float t = (h - at.hpl0) / at.hpl1;
+      acc[6] += 0.5f * t * t - at.hpl2;
+    }
+    for (int m = wi; m < M; m += W) {  // MCS rows (energy_mcs_v1.C:5-29)
+      const float d = gd_sum_strided(part_mcs + (size_t)m * nL * Rp + r, nL, (size_t)Rp);
+      const float2 row = mcsrow[m];
+      acc[7] += row.x * sqrtf(d / row.y);
+    }
   }
 #pragma unroll
   for (int c = 0; c < 8; ++c) red[wave][c][lane] = acc[c];
