@@ -693,6 +693,7 @@ __global__ __launch_bounds__(256) void gd_reduce_kernel(const GdJob *__restrict_
 // proposal if it is accepted, from the current pose if not) are computed by
 // waves 1-3 while the slices are summed, and wave 0 only selects.
 #define GD_FW 16
+#define GD_FPA 4  // conformer atoms per wave prefetched for the transform
 __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict__ jobs,
                                                           const GdBlock *__restrict__ blocks,
                                                           int mode, int launch_step,
@@ -746,6 +747,24 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
 #pragma unroll
     for (int i = 0; i < 6; ++i) base[i] = src[(size_t)i * Rp + r];
   }
+  // loads that do not wait for the energies, issued now: wave 0's replica
+  // state, and every wave's first GD_FPA conformer atoms for the transform
+  const GD_AS_GLOBAL float *prop_c = gd_g(J.prop), *en_c = gd_g(J.en);
+  float m[6], en9 = 0.0f, mbv = 0.0f;
+  if (wave == 0) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) m[i] = prop_c[(size_t)i * Rp + r];
+    if (move) {
+      en9 = en_c[(size_t)9 * Rp + r];
+      mbv = gd_g(J.minus_beta)[gd_g(J.tmpi)[r]];
+    }
+  }
+  const int L = J.L;
+  const GD_AS_GLOBAL float4 *lc = gd_g(J.lig0) + (size_t)gd_g(J.ligi)[r] * L;
+  float4 pa[GD_FPA];
+  if (move)
+#pragma unroll
+    for (int q = 0; q < GD_FPA; ++q) pa[q] = lc[min(wave + q * GD_FW, L - 1)];
   __syncthreads();
 
   if (move && (wave == 1 || wave == 3)) {  // rotation of each candidate next proposal
@@ -770,9 +789,6 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
       for (int w = 0; w < GD_FW; ++w) v += red[w][k][lane];
       s[k] = v;
     }
-    float m[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) m[i] = prop[(size_t)i * Rp + r];
     // e_s assembly (kernel_cpp_montecarlo_soa.C:318-324), KDE/MCS/DST (:328-359)
     float e[10];
     e[0] = s[0];
@@ -829,7 +845,7 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
       }
       // Metropolis (:382-392); the initial pass draws u = 0
       const float u = u_s[lane];
-      const float x = (tot - en[(size_t)9 * Rp + r]) * gd_g(J.minus_beta)[gd_g(J.tmpi)[r]];
+      const float x = (tot - en9) * mbv;
       const bool a = (u == 0.0f) ? (x >= GD_EXP_UNDERFLOW) : (u < expf(x));
       acc_s[lane] = live && a;  // padding lanes keep their pose
       if (live) {
@@ -856,7 +872,17 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
   t[0] = xf[c][9][lane];
   t[1] = xf[c][10][lane];
   t[2] = xf[c][11][lane];
-  gd_transform_atoms(J, r, R, t, wave, GD_FW);
+  // the transform of gd_transform_atoms, the first atoms from the prefetch
+  GD_AS_GLOBAL float *co = gd_g(J.coords);
+  auto put = [&](int l, const float4 a) {
+    co[(size_t)(3 * l + 0) * Rp + r] = R[0] * a.x + R[1] * a.y + R[2] * a.z + t[0];
+    co[(size_t)(3 * l + 1) * Rp + r] = R[3] * a.x + R[4] * a.y + R[5] * a.z + t[1];
+    co[(size_t)(3 * l + 2) * Rp + r] = R[6] * a.x + R[7] * a.y + R[8] * a.z + t[2];
+  };
+#pragma unroll
+  for (int q = 0; q < GD_FPA; ++q)
+    if (wave + q * GD_FW < L) put(wave + q * GD_FW, pa[q]);
+  for (int l = wave + GD_FPA * GD_FW; l < L; l += GD_FW) put(l, lc[l]);
 }
 
 // ---------------------------------------------------------------------------
