@@ -198,10 +198,11 @@ gd_status exchange_upload(const BatchLists &B, const ExchPoint &xp, hipStream_t 
 // One MC step of every job in the lists (pair + finish), optional exchange point.
 gd_status batch_step(const BatchLists &B, int lc, int grid, int launch_step, const int *base,
                      const ExchPoint *xp, hipStream_t s, KTimer *tp) {
-  if (tp && tp->begin(s) != GD_OK) return GD_ERR_HIP;
+  const bool timed = tp && launch_step % kTimedEvery == 0;
+  if (timed && tp->begin(s) != GD_OK) return GD_ERR_HIP;
   hipError_t e = launch_pair_wq(lc, B.d_jobs, B.d_items, B.n_items, B.d_wq, grid, s);
   if (e != hipSuccess) return hip_fail(e, "pair kernel (queue) launch");
-  if (tp && tp->end(s) != GD_OK) return GD_ERR_HIP;
+  if (timed && tp->end(s) != GD_OK) return GD_ERR_HIP;
   hipLaunchKernelGGL(gd_reduce_kernel, dim3(B.n_blocks * GD_RK), dim3(256), 0, s, B.d_jobs, B.d_blocks);
   hipLaunchKernelGGL(gd_finish_kernel, dim3(B.n_blocks), dim3(1024), 0, s, B.d_jobs, B.d_blocks,
                      GD_MODE_STEP, launch_step, base, B.d_wq);

@@ -149,12 +149,18 @@ __device__ __forceinline__ void gd_pair_one(
       asm volatile("");
       // -x = log2(e)/2 r^2 + pB r + pC
       const float nx = __builtin_fmaf(pB, rr, __builtin_fmaf(pmf_a, r2, pC));
-      // distance-dependent electrostatics (:72-78), charge applied per atom:
-      // el1 g = poly(r) for s = el1 r < 1, else 1/r
-      const float poly = __builtin_fmaf(__builtin_fmaf(k.e3, rr, k.e2), r2, k.e0);
-      g = rr < k.inv_el1 ? poly : ri;
       const float t = gd_rcp(1.0f + gd_exp2(nx));
       epmf = __builtin_fmaf(-pmf1, near ? t : 0.0f, epmf);
+      // distance-dependent electrostatics (:72-78), charge applied per atom:
+      // el1 g = poly(r) for s = el1 r < 1, else 1/r.  s < 1 is a close
+      // contact (r < 2 A with the published el1 = 0.5), so the polynomial is
+      // evaluated only when some lane of the wave is that close
+      const bool close = rr < k.inv_el1;
+      if (__ballot(close) != 0) {
+        asm volatile("");
+        const float poly = __builtin_fmaf(__builtin_fmaf(k.e3, rr, k.e2), r2, k.e0);
+        g = close ? poly : ri;
+      }
     }
     elej = __builtin_fmaf(pele, g, elej);
     // pocket-specific potential (:100-107): r <= pmf0 decided on r^2
@@ -257,25 +263,32 @@ __device__ __forceinline__ void gd_pair_atoms_load(
 // tile holds the same mix of near and far points and waves cost the same.
 template <int LC, int CLS>
 __device__ __forceinline__ void gd_pair_class(
-    const GD_AS_CONST GdJob &J, const int lo, const int hi, const int b,
-    const GD_AS_GLOBAL float4 *__restrict__ pc, const GdPairConst &k, const float pmf_a,
+    const GD_AS_CONST GdJob &J, const int lo, const int hi, const int b, const unsigned pcoff,
+    const GdPairConst &k, const float pmf_a,
     const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC], const unsigned (&trow)[LC],
     float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf, float &epsp, float &ehdb) {
   const int P = J.P, nP = J.nP;
   const int p0 = lo + b;
   if (p0 >= hi) return;
-  const GD_AS_GLOBAL float4 *__restrict__ pattr = gd_g(J.pattr);
+  const GD_AS_GLOBAL char *__restrict__ pcb = (const GD_AS_GLOBAL char *)gd_g(J.pcoord);
+  const GD_AS_GLOBAL char *__restrict__ pab = (const GD_AS_GLOBAL char *)gd_g(J.pattr);
   const GD_AS_CONST char *__restrict__ ptab = (const GD_AS_CONST char *)gd_c(J.ptab);
   const GD_AS_CONST float *__restrict__ pspv = gd_c(J.pspv);
   // The point attributes (uniform) go through the vector memory path and are
   // prefetched one point ahead with the coordinates: as scalar loads they
-  // stalled the wave.  The address is uniform (SGPRs) plus a zero VGPR offset
-  // whose value the compiler cannot see, which keeps the load on the vector
-  // path at no per-point VALU cost.
-  int zo = 0;
-  asm("" : "+v"(zo));
+  // stalled the wave.  Both loads take a uniform base in SGPRs plus a 32-bit
+  // VGPR offset (zero for the attributes, the lane's protein conformer for the
+  // coordinates), redeclared opaque at each load so that the compiler does not
+  // fold base and offset into a 64-bit VGPR address: the point index then
+  // costs scalar adds only, no per-point VALU.
+  unsigned zo = 0, vo = pcoff;
   auto attr = [&](int q) {
-    return *(const GD_AS_GLOBAL float4 *)((const GD_AS_GLOBAL char *)(pattr + q) + zo);
+    asm("" : "+v"(zo));
+    return *(const GD_AS_GLOBAL float4 *)(pab + (size_t)q * 16 + zo);
+  };
+  auto coord = [&](int q) {
+    asm("" : "+v"(vo));
+    return *(const GD_AS_GLOBAL float4 *)(pcb + (size_t)q * 16 + vo);
   };
   constexpr bool kRows = LC == 4;  // rows of 4 atoms fit the scalar registers
   float rw[kRows ? LC : 1][GD_ROW];
@@ -301,13 +314,13 @@ __device__ __forceinline__ void gd_pair_class(
   // Two points per iteration, each prefetching the next point into the other
   // register set, so the loop carries no register copies.  The prefetch index
   // is clamped into the class (the last one reloads a point, unused).
-  float4 c0 = pc[p0], a0 = attr(p0);
+  float4 c0 = coord(p0), a0 = attr(p0);
   int p = p0;
   for (; p + nP < hi; p += 2 * nP) {
-    const float4 c1 = pc[p + nP], a1 = attr(p + nP);
+    const float4 c1 = coord(p + nP), a1 = attr(p + nP);
     point(p, c0, a0);
     const int q = min(p + 2 * nP, hi - 1);
-    c0 = pc[q];
+    c0 = coord(q);
     a0 = attr(q);
     point(p + nP, c1, a1);
   }
@@ -318,7 +331,7 @@ __device__ __forceinline__ void gd_pair_class(
 // chunk contribute exactly zero).
 template <int LC>
 __device__ __forceinline__ void gd_pair_points(
-    const GD_AS_CONST GdJob &J, const int b, const GD_AS_GLOBAL float4 *__restrict__ pc,
+    const GD_AS_CONST GdJob &J, const int b, const unsigned pc,
     const GdPairConst &k, const float (&lx)[LC], const float (&ly)[LC], const float (&lz)[LC],
     const unsigned (&trow)[LC], float (&ele)[LC], float (&hpc)[LC], float &evdw, float &epmf,
     float &epsp, float &ehdb) {
@@ -361,7 +374,8 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
     ele[j] = 0.0f;
     hpc[j] = 0.0f;
   }
-  const GD_AS_GLOBAL float4 *__restrict__ pc = gd_g(J.pcoord) + (size_t)gd_g(J.prti)[r] * P;
+  // byte offset of this lane's protein conformer in the point coordinates
+  const unsigned pc = (unsigned)gd_g(J.prti)[r] * (unsigned)P * 16u;
   GdPairConst k;
   k.lj0 = J.lj0;
   k.lj01 = J.lj0 * J.lj1;

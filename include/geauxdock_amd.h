@@ -148,9 +148,9 @@ typedef struct {
 
 typedef struct {
   double ms_total;          /* wall time of the MC loop (host clock) */
-  double ms_pair_kernel;    /* summed HIP-event time of the pair-energy kernel */
-  double ms_finish_kernel;  /* summed HIP-event time of the finish kernel */
-  int n_pair_launch;        /* pair-kernel launches timed */
+  double ms_pair_kernel;    /* summed HIP-event time of the timed pair-energy launches */
+  double ms_finish_kernel;  /* reserved, 0: the tail kernels are not event-timed */
+  int n_pair_launch;        /* pair-kernel launches timed: those of MC steps 0, 16, 32, ... */
   double pair_evals;        /* ligand-atom x protein-point evaluations done */
   double replica_steps;     /* replica x MC steps done (initial pass excluded) */
 } gd_stats;
