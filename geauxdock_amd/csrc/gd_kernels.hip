@@ -106,20 +106,6 @@ __device__ __forceinline__ void gd_pair_one(
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     const float ri = gd_rsq(r2);
     const float rr = r2 * ri;
-    // hydrophobic contact (energy_prt_v1.C:44-50):
-    // hpp (1 - r^2 (a - b r^2 + c r^4 - d r^6)) for r^2 <= 81, on the lanes
-    // within 9 A only (exec-masked; skipped when no lane is).  The empty
-    // volatile asm keeps the branch: without it the compiler evaluates the
-    // term for every pair and selects
-    if (CLS & 1) {
-      if (r2 <= 81.0f) {
-        asm volatile("");
-        float w = __builtin_fmaf(-GD_HPD, r2, GD_HPC);
-        w = __builtin_fmaf(w, r2, -GD_HPB);
-        w = __builtin_fmaf(w, r2, GD_HPA);
-        hpcj = __builtin_fmaf(phpp, __builtin_fmaf(-r2, w, 1.0f), hpcj);
-      }
-    }
     // soft Lennard-Jones (:61-64): (p1 - p2) / (p1 lj0 (1 + lj1 r^2) + 1) with
     // p1 = p1a / r^9, p2 = p2a / r^6, multiplied through by r^9 / p1a:
     // (1 - q r^3) / (lj0 (1 + lj1 r^2) + w r^9), q = p2a / p1a, w = 1 / p1a
@@ -161,25 +147,41 @@ __device__ __forceinline__ void gd_pair_one(
         const float poly = __builtin_fmaf(__builtin_fmaf(k.e3, rr, k.e2), r2, k.e0);
         g = close ? poly : ri;
       }
-    }
-    elej = __builtin_fmaf(pele, g, elej);
-    // pocket-specific potential (:100-107): r <= pmf0 decided on r^2
-    // against the exact float threshold of sqrtf(r^2) <= pmf0; the value is
-    // loaded only when some lane is in contact (exec-masked branch)
-    if (CLS & 2) {
-      if (r2 <= fabsf(pthr)) {
+      // The terms below vanish beyond the row's threshold too (it is raised
+      // to cover their ranges), so far waves skip them.
+      // hydrophobic contact (energy_prt_v1.C:44-50):
+      // hpp (1 - r^2 (a - b r^2 + c r^4 - d r^6)) for r^2 <= 81, on the lanes
+      // within 9 A only (exec-masked; skipped when no lane is).  The empty
+      // volatile asm keeps the branch: without it the compiler evaluates the
+      // term for every pair and selects
+      if (CLS & 1) {
+        if (r2 <= 81.0f) {
+          asm volatile("");
+          float w = __builtin_fmaf(-GD_HPD, r2, GD_HPC);
+          w = __builtin_fmaf(w, r2, -GD_HPB);
+          w = __builtin_fmaf(w, r2, GD_HPA);
+          hpcj = __builtin_fmaf(phpp, __builtin_fmaf(-r2, w, 1.0f), hpcj);
+        }
+      }
+      // pocket-specific potential (:100-107): r <= pmf0 decided on r^2
+      // against the exact float threshold of sqrtf(r^2) <= pmf0; the value is
+      // loaded only when some lane is in contact (exec-masked branch)
+      if (CLS & 2) {
+        if (r2 <= fabsf(pthr)) {
+          asm volatile("");
+          epsp += pspv[(size_t)prow * P + p];
+        }
+      }
+      // hydrogen bond (:116-121) as c exp2(-(r c - hdb0 c)^2), c = hdb1 sqrt(log2(e)/2),
+      // summed without the 1/sqrt(log2(e)/2) factor; pairs with hdb0 <= 0.1 skip
+      // the term on a wave-uniform branch
+      if (hflag < 0) {
         asm volatile("");
-        epsp += pspv[(size_t)prow * P + p];
+        const float h = __builtin_fmaf(rr, hdb1c, nhdb0c);
+        ehdb = __builtin_fmaf(hdb1c, gd_exp2(-(h * h)), ehdb);
       }
     }
-    // hydrogen bond (:116-121) as c exp2(-(r c - hdb0 c)^2), c = hdb1 sqrt(log2(e)/2),
-    // summed without the 1/sqrt(log2(e)/2) factor; pairs with hdb0 <= 0.1 skip
-    // the term on a wave-uniform branch
-    if (hflag < 0) {
-      asm volatile("");
-      const float h = __builtin_fmaf(rr, hdb1c, nhdb0c);
-      ehdb = __builtin_fmaf(hdb1c, gd_exp2(-(h * h)), ehdb);
-    }
+    elej = __builtin_fmaf(pele, g, elej);
 }
 
 // Rows of the LC chunk atoms for one point type, loaded into scalar registers.

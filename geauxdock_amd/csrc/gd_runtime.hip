@@ -356,7 +356,21 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       const double c = (double)e->hdb1[lt][pt] * hdb_c;
       o[6] = hb ? (float)c : 0.0f;
       o[7] = hb ? (float)(-(double)e->hdb0[lt][pt] * c) : 0.0f;
-      o[8] = far_r2_threshold(pmf0, e->el1);
+      // The pair loop evaluates the hydrophobic (r^2 <= 81), psp (r^2 <= |o[5]|)
+      // and H-bond terms inside its near-field branch only: the threshold also
+      // covers their ranges, so a wave whose lanes are all beyond it skips
+      // exact zeros.  H-bond: exp2(-h^2) is exactly 0 in FP32 for h^2 >= 150;
+      // h^2 >= 160 from r = hdb0 + sqrt(160) / c on.
+      double r2n = far_r2_threshold(pmf0, e->el1);
+      r2n = std::max(r2n, 81.0 * (1.0 + 1e-6));
+      if (thr >= 0.0f) r2n = std::max(r2n, (double)thr * (1.0 + 1e-6));
+      if (hb && c > 0.0) {
+        const double rh = (double)e->hdb0[lt][pt] + std::sqrt(160.0) / c;
+        r2n = std::max(r2n, rh * rh * (1.0 + 1e-5));
+      }
+      float f = (float)r2n;
+      if ((double)f < r2n) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+      o[8] = f;
     }
   }
   // Protein points, kept in source order: the device order depends on the tile
