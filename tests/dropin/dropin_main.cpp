@@ -1,5 +1,5 @@
 // dropin_main.cpp -- TEST PROGRAM: the reference GPU driver's call sequence
-// (src/main_gpu/geauxdock.C:59-104: SetDevice, DeviceAlloc, InitCurand, then
+// (src/main_gpu/geauxdock.C:82-116: SetDevice, DeviceAlloc, InitCurand, then
 // per complex SetParaT, CopyH2D, Dock; DeviceFree at the end) run against
 // libgeauxdock_amd.so through include/geauxdock_dropin.h, on a complex filled
 // into the reference's own Complex layout from the library's loaders.
