@@ -13,8 +13,9 @@ steps of all 1,760 replicas + one exchange point), so the default --steps 100 is
 
 Prints one JSON line (value = replica-steps/s).  The roofline object covers
 the dominant kernel (gd_pair_kernel<LC>, LC ligand atoms per lane): algorithmic FP32 flops per launch /
-its mean launch time from HIP events recorded around every launch on the
-job's stream.  cpu_baseline times the CPU oracle (a restatement of the
+its mean launch time from HIP events recorded on the job's stream around
+the launch of every 16th MC step (625 launches over the job; the library
+samples so that the records, about 4 us each, stay under 1% of the run).  cpu_baseline times the CPU oracle (a restatement of the
 reference CPU backend) on a bounded sample of the same workload.
 """
 import argparse
