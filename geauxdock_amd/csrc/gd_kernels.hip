@@ -717,7 +717,7 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
                                                           int *__restrict__ wq) {
   if (wq && blockIdx.x == 0 && threadIdx.x == 0) *wq = 0;  // next pair launch's queue
   if (step_base) launch_step += *step_base;  // graph replays: base step in device memory
-  __shared__ float red[GD_FW][8][64];
+  __shared__ float red[8][64];
   __shared__ float xf[2][12][64];  // next transform if this step is accepted [0] / rejected [1]
   __shared__ float inc_s[6][64];
   __shared__ float u_s[64];
@@ -730,17 +730,18 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
   const int step = launch_step - J.step0;  // this job's MC step
   const bool move = mode != GD_MODE_EVAL;
 
-  // (1) the reduce kernel's slices: wave w reads slices w, w + 16, ...
-  {
-    const GD_AS_GLOBAL float *fp = gd_g(J.fin_part) + (size_t)(bk.rb / 64) * GD_RK * 8 * 64;
-    float acc[8];
+  // (1) the reduce kernel's slices: wave c < 8 sums value c over the GD_RK
+  // slices in slice order (all loads issued first)
+  if (wave < 8) {
+    const GD_AS_GLOBAL float *fp =
+        gd_g(J.fin_part) + (size_t)(bk.rb / 64) * GD_RK * 8 * 64 + (size_t)wave * 64 + lane;
+    float v[GD_RK];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[c] = 0.0f;
-    for (int k = wave; k < GD_RK; k += GD_FW)
+    for (int k = 0; k < GD_RK; ++k) v[k] = fp[(size_t)k * 8 * 64];
+    float acc = 0.0f;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) acc[c] += fp[((size_t)k * 8 + c) * 64 + lane];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) red[wave][c][lane] = acc[c];
+    for (int k = 0; k < GD_RK; ++k) acc += v[k];
+    red[wave][lane] = acc;
   }
   // (2) off the acceptance path: wave 1 the next step's increments, wave 2 this
   // step's accept draw (the initial pass draws u = 0, a forced accept)
@@ -765,14 +766,19 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
   }
   // loads that do not wait for the energies, issued now: wave 0's replica
   // state, and every wave's first GD_FPA conformer atoms for the transform
-  const GD_AS_GLOBAL float *prop_c = gd_g(J.prop), *en_c = gd_g(J.en);
-  float m[6], en9 = 0.0f, mbv = 0.0f;
+  const GD_AS_GLOBAL float *prop_c = gd_g(J.prop), *en_c = gd_g(J.en), *mm_c = gd_g(J.mm);
+  float m[6], m0[6], en9 = 0.0f, mbv = 0.0f, best0 = 0.0f;
+  int nacc0 = 0;
   if (wave == 0) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) m[i] = prop_c[(size_t)i * Rp + r];
     if (move) {
       en9 = en_c[(size_t)9 * Rp + r];
       mbv = gd_g(J.minus_beta)[gd_g(J.tmpi)[r]];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) m0[i] = mm_c[(size_t)i * Rp + r];
+      best0 = gd_g(J.best)[r];
+      nacc0 = gd_g(J.nacc)[r];
     }
   }
   const int L = J.L;
@@ -799,12 +805,7 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
     GD_AS_GLOBAL float *prop = gd_g(J.prop), *en = gd_g(J.en), *mm = gd_g(J.mm);
     float s[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      float v = 0.0f;
-#pragma unroll
-      for (int w = 0; w < GD_FW; ++w) v += red[w][k][lane];
-      s[k] = v;
-    }
+    for (int k = 0; k < 8; ++k) s[k] = red[k][lane];
     // e_s assembly (kernel_cpp_montecarlo_soa.C:318-324), KDE/MCS/DST (:328-359)
     float e[10];
     e[0] = s[0];
@@ -869,14 +870,15 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
         if (a) {
           for (int i = 0; i < 10; ++i) en[(size_t)i * Rp + r] = e[i];
           for (int i = 0; i < 6; ++i) mm[(size_t)i * Rp + r] = m[i];
-          if (mode == GD_MODE_STEP) gd_g(J.nacc)[r] += 1;
-          GD_AS_GLOBAL float *best = gd_g(J.best);
-          best[r] = fminf(best[r], tot);
+          if (mode == GD_MODE_STEP) gd_g(J.nacc)[r] = nacc0 + 1;
+          gd_g(J.best)[r] = fminf(best0, tot);
         }
       }
-      // the next step's proposal, from the accepted pose
+      // the next step's proposal, from the accepted pose (padding lanes keep
+      // theirs: they never accept)
+      const bool took = live && a;
 #pragma unroll
-      for (int i = 0; i < 6; ++i) prop[(size_t)i * Rp + r] = inc_s[i][lane] + mm[(size_t)i * Rp + r];
+      for (int i = 0; i < 6; ++i) prop[(size_t)i * Rp + r] = inc_s[i][lane] + (took ? m[i] : m0[i]);
     }
   }
   if (!move) return;
