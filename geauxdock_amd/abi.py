@@ -153,6 +153,7 @@ EXTRA_PROTOTYPES = {
     "gd_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int,
                          C.POINTER(C.c_uint32)]),
     "gd_far_r2": (C.c_float, [C.c_float, C.c_float]),
+    "gd_near_r2": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
 }
 
 

@@ -161,6 +161,26 @@ float far_r2_threshold(double pmf0, double el1) {
   return (double)f >= r * r ? f : std::nextafter(f, std::numeric_limits<float>::infinity());
 }
 
+// The row threshold of the pair loop's near branch: the far-field threshold
+// raised to cover the terms that branch also holds, so that a wave whose lanes
+// are all beyond it skips exact zeros only:
+//   hydrophobic, r^2 <= 81; psp, r^2 <= psp_thr (negative: no psp term);
+//   H-bond (hb), c exp2(-h^2) with h = (r - hdb0) c: exp2 is exactly 0 in FP32
+//   for h^2 >= 150, and h^2 >= 160 from r = hdb0 + sqrt(160) / c on.
+static float near_r2_threshold(double pmf0, double el1, float psp_thr, bool hb, double hdb0,
+                               double c) {
+  double r2n = far_r2_threshold(pmf0, el1);
+  r2n = std::max(r2n, 81.0 * (1.0 + 1e-6));
+  if (psp_thr >= 0.0f) r2n = std::max(r2n, (double)psp_thr * (1.0 + 1e-6));
+  if (hb && c > 0.0) {
+    const double rh = hdb0 + std::sqrt(160.0) / c;
+    r2n = std::max(r2n, rh * rh * (1.0 + 1e-5));
+  }
+  float f = (float)r2n;
+  if ((double)f < r2n) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+  return f;
+}
+
 // ---------------------------------------------------------------------------
 // Tiling: ligand chunk size lc (kernel template), protein tile ptile.  The
 // default aims for ~22 waves per SIMD over the whole launch; it depends only
@@ -356,21 +376,7 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       const double c = (double)e->hdb1[lt][pt] * hdb_c;
       o[6] = hb ? (float)c : 0.0f;
       o[7] = hb ? (float)(-(double)e->hdb0[lt][pt] * c) : 0.0f;
-      // The pair loop evaluates the hydrophobic (r^2 <= 81), psp (r^2 <= |o[5]|)
-      // and H-bond terms inside its near-field branch only: the threshold also
-      // covers their ranges, so a wave whose lanes are all beyond it skips
-      // exact zeros.  H-bond: exp2(-h^2) is exactly 0 in FP32 for h^2 >= 150;
-      // h^2 >= 160 from r = hdb0 + sqrt(160) / c on.
-      double r2n = far_r2_threshold(pmf0, e->el1);
-      r2n = std::max(r2n, 81.0 * (1.0 + 1e-6));
-      if (thr >= 0.0f) r2n = std::max(r2n, (double)thr * (1.0 + 1e-6));
-      if (hb && c > 0.0) {
-        const double rh = (double)e->hdb0[lt][pt] + std::sqrt(160.0) / c;
-        r2n = std::max(r2n, rh * rh * (1.0 + 1e-5));
-      }
-      float f = (float)r2n;
-      if ((double)f < r2n) f = std::nextafter(f, std::numeric_limits<float>::infinity());
-      o[8] = f;
+      o[8] = near_r2_threshold(pmf0, e->el1, thr, hb, e->hdb0[lt][pt], hb ? c : 0.0);
     }
   }
   // Protein points, kept in source order: the device order depends on the tile
@@ -808,6 +814,14 @@ const char *gd_version(void) { return "geauxdock_amd 0.1 (gfx950)"; }
 
 // The pair kernel's far-field r^2 threshold for (pmf0, el1) (diagnostics, tests).
 float gd_far_r2(float pmf0, float el1) { return gdr::far_r2_threshold(pmf0, el1); }
+
+// The row threshold of the pair loop's near branch for a (ligand type, point
+// type) pair of the force field (diagnostics, tests): as the device row holds it.
+float gd_near_r2(float pmf0, float el1, float hdb0, float hdb1) {
+  const bool hb = hdb0 > 0.1f;
+  const double c = (double)hdb1 * std::sqrt(0.5 * 1.44269504088896340736);
+  return gdr::near_r2_threshold(pmf0, el1, gdr::sqrt_threshold(pmf0), hb, hdb0, hb ? c : 0.0);
+}
 
 void gd_philox(const uint32_t ctr[4], const uint32_t key[2], int rounds, uint32_t out[4]) {
   const gd_u32x4 r = gd_philox4x32(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1], rounds);

@@ -431,3 +431,37 @@ def test_far_field_threshold(el1):
             assert el1 * rp >= 1.0
         rs = 0.98 * r
         assert log2e * (6.0 - rs / 2.0) * (rs - pmf0) > -26.0 or el1 * rs < 1.0
+
+
+def test_near_branch_threshold(ene):
+    """The row threshold of the pair loop's near branch (library, FP64 host code),
+    for every (ligand type, point type) row of the published force field: it is
+    at least the far-field threshold, and from it on the terms the branch also
+    holds are exactly zero in the kernel's FP32 arithmetic (hydrophobic r^2 <= 81,
+    psp sqrtf(r^2) <= pmf0, H-bond c exp2(-h^2) with h = fma(r, c, -hdb0 c)), so
+    a wave whose lanes are all beyond it may skip the branch bit-exactly."""
+    f32 = np.float32
+    k = math.sqrt(0.5 / math.log(2.0))
+    pmf0 = np.array(ene.pmf0, np.float32)
+    hdb0 = np.array(ene.hdb0, np.float32)
+    hdb1 = np.array(ene.hdb1, np.float32)
+    n_hb = 0
+    for lt in range(pmf0.shape[0]):
+        for pt in range(pmf0.shape[1]):
+            p0, h0, h1 = float(pmf0[lt, pt]), float(hdb0[lt, pt]), float(hdb1[lt, pt])
+            if not (np.isfinite(p0) and np.isfinite(h0) and np.isfinite(h1)):
+                continue
+            r2n = g.lib().gd_near_r2(p0, ene.el1, h0, h1)
+            assert r2n >= g.lib().gd_far_r2(p0, ene.el1)
+            r2 = (f32(r2n) * (1.0 + np.linspace(0.0, 8.0, 2001))).astype(f32)
+            r2 = np.concatenate([[f32(r2n)], r2])
+            assert np.all(r2 > f32(81.0))
+            assert np.all(np.sqrt(r2).astype(f32) > f32(p0))          # no psp contact
+            if f32(h0) > f32(0.1):
+                n_hb += 1
+                c = f32(h1 * k)
+                n0 = f32(-float(h0) * float(c))
+                rr = (r2 * (f32(1.0) / np.sqrt(r2).astype(f32))).astype(f32)
+                h = (rr.astype(np.float64) * float(c) + float(n0)).astype(f32)
+                assert np.all(np.exp2(-(h * h)).astype(f32) == 0.0), (lt, pt)
+    assert n_hb > 0
