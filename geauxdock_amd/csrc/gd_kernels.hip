@@ -1,20 +1,24 @@
 // gd_kernels.hip -- hand-written gfx950 kernels of the Monte Carlo hot path.
 //
-// One MC step of every replica is two launches:
-//   gd_pair_kernel   one wave per work item (64 replicas x one ligand-atom
-//                    chunk x one protein tile): PRT pair terms, the item's
-//                    slice of the KDE anchors and of the MCS rows; writes
-//                    partial sums.  Lanes are replicas, so protein points,
-//                    force-field rows, KDE anchors and MCS targets are
-//                    wave-uniform scalar loads and the inner loop is pure FP32
-//                    VALU on ligand coordinates held in VGPRs.
-//   gd_finish_kernel one workgroup per 64-replica block: reduces the partials
-//                    (waves split the slots, LDS combines them), applies the
-//                    non-linear per-atom/per-row terms, normalisation and
-//                    weights, the Metropolis test, records, and proposes +
-//                    transforms the next move (Philox4x32-10, counter-based).
+// One MC step of every replica is three launches:
+//   gd_pair_kernel   one 8-wave work group per item (64 replicas x one
+//                    ligand-atom chunk x eight protein tiles, one per wave):
+//                    PRT pair terms, the tiles' slices of the KDE anchors and
+//                    of the MCS rows; the group sums its tiles in LDS and
+//                    writes partial sums.  Lanes are replicas, so protein
+//                    attributes, force-field rows, KDE anchors and MCS targets
+//                    are wave-uniform scalar loads and the inner loop is pure
+//                    FP32 VALU on ligand coordinates held in VGPRs.
+//   gd_reduce_kernel 16 work groups per 64-replica block: sum the partial
+//                    sums and apply the non-linear per-atom (hpc) and per-row
+//                    (MCS) terms; one slice of 8 values per group.
+//   gd_finish_kernel one work group per 64-replica block: sums the slices,
+//                    normalises and weights, records, runs the Metropolis
+//                    test, and proposes + transforms the next move
+//                    (Philox4x32-10, counter-based).
 // plus gd_transform_kernel (pose -> ligand coordinates, used before the first
-// pass and for energy evaluation) and gd_exchange_kernel (temperature swaps).
+// pass and for energy evaluation), gd_exchange_kernel (temperature swaps) and
+// the pose-swap kernels gd_swap_propose_kernel / gd_swap_accept_kernel.
 //
 // The reference computes the same quantities in one CUDA block per replica
 // (backend_gpu/kernel_cuda_l1_montecarlo.cu:29-536); here the pair loop
