@@ -3,8 +3,8 @@
 // tiling autotuner, and the continuous-batching serving loop.
 //
 // A "batch" concatenates the device structs of its jobs into one array, and
-// their work items and 64-replica blocks into one list each, so one pair
-// launch and one finish launch advance every job by one MC step.  Each job
+// their work items and 64-replica blocks into one list each, so one pair,
+// one reduce and one finish launch advance every job by one MC step.  Each job
 // keeps its own tiling, replica order, Philox stream (seed, job_id) and step
 // counter (launch step - step0), so a job computes bit-identical results
 // alone or inside any batch.  The reference has no counterpart: its MPI

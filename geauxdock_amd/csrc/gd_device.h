@@ -11,7 +11,7 @@
 //  * only the protein coordinates depend on a lane (its protein conformer):
 //    float4 [n_prt][P], <= a handful of distinct rows per wave;
 //  * partial sums are [slot][Rp] so both the pair kernel (writer) and the
-//    finish kernel (reader) are coalesced across lanes.
+//    reduce kernel (reader) are coalesced across lanes.
 #pragma once
 #include <stdint.h>
 

@@ -7,8 +7,8 @@
 // prepair.C:12-135): CopyH2D of a 6.3 MiB Complex per job, one launch for
 // the initial energies, one launch + blocking D2H of records per dump.  Here
 // a job's tables are packed once into device memory sized to the job, the
-// replica state never leaves the device, and every MC step is a pair-kernel
-// + finish-kernel launch on the job's stream.
+// replica state never leaves the device, and every MC step is a pair-,
+// reduce- and finish-kernel launch on the job's stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
