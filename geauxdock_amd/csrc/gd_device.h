@@ -94,6 +94,7 @@ struct GdJob {
   float inv_L;
   float pmf_all;          // pmf1 summed over every (atom, point) pair of the job
   float center[3], pocket[3];
+  float cp[3];            // center - pocket, rounded once from FP64 (DST term)
   float a_para[10], b_para[10], w[10];
   float move_scale[6];
   int move_mode;

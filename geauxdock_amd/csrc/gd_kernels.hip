@@ -824,7 +824,10 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
       float d2 = 0.0f;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const float d = J.center[i] + m[i] - J.pocket[i];
+        // (center + m) - pocket restated as m + (center - pocket): the loaders
+        // put the ligand centre on the pocket centre, so this is m exactly, where
+        // the reference's order rounds m to the ulp of the centre coordinate
+        const float d = m[i] + J.cp[i];
         d2 += d * d;
       }
       e[8] = sqrtf(d2);

@@ -500,7 +500,11 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
   g.kde2l = e->kde2 * 1.44269504088896340736f;
   g.kde3_inv = 1.0f / e->kde3;
   g.inv_L = 1.0f / (float)L;
-  for (int i = 0; i < 3; ++i) g.center[i] = d->lig_center[i], g.pocket[i] = d->pocket_center[i];
+  for (int i = 0; i < 3; ++i) {
+    g.center[i] = d->lig_center[i];
+    g.pocket[i] = d->pocket_center[i];
+    g.cp[i] = (float)((double)d->lig_center[i] - (double)d->pocket_center[i]);
+  }
   for (int i = 0; i < 10; ++i) g.a_para[i] = e->a_para[i], g.b_para[i] = e->b_para[i], g.w[i] = e->w[i];
 
   // device memory: constant tables + replica state

@@ -310,10 +310,13 @@ void energy(const Job &J, int prt, const float m[6], const T *xf, const T *yf, c
     sc[7] = std::max(1.0, fabs((double)es[7]));
     (void)s_abs;
   }
-  {  // DST, kernel_cpp_montecarlo_soa.C:348-359
+  {  // DST, kernel_cpp_montecarlo_soa.C:348-359; the literal mode keeps the
+     // reference's order, the other modes compute m + (center - pocket) with the
+     // difference rounded once, as the kernel does
     T dst = 0;
     for (int i = 0; i < 3; ++i) {
-      const T dd = (T)d->lig_center[i] + (T)m[i] - (T)d->pocket_center[i];
+      const T dd = literal ? (T)d->lig_center[i] + (T)m[i] - (T)d->pocket_center[i]
+                           : (T)m[i] + (T)((double)d->lig_center[i] - (double)d->pocket_center[i]);
       dst += dd * dd;
     }
     es[8] = t_sqrt(dst);

@@ -9,6 +9,8 @@ oracle is replayed at the GPU's own transformed coordinates so that the
 discontinuous psp cutoff decides on identical distances; the transform is
 checked separately.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -17,6 +19,9 @@ from tests.oracle_ffi import random_poses, rel_err
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-5
+# pose counts of the random-pose parity tests; GD_PARITY_POSES raises them for
+# a longer sweep (the defaults keep the suite to about a minute)
+N_POSES = int(os.environ.get("GD_PARITY_POSES", "0"))
 GOLDEN = np.array([0.6434, -0.0367, -0.2079, -0.1845, 0.8521, -0.8880, 0.0523, 0.1744, -1.0000,
                    0.7735])
 
@@ -62,7 +67,7 @@ def test_golden_line(golden_job, golden_dev, oracle):
 
 
 def test_golden_random_poses(golden_job, golden_dev, oracle):
-    conf, pose = random_poses(golden_job, 512, seed=11)
+    conf, pose = random_poses(golden_job, max(512, N_POSES), seed=11)
     _parity(golden_job, golden_dev, oracle, conf, pose, "1a07C1 x512")
 
 
@@ -77,7 +82,7 @@ def test_golden_all_conformers_native(golden_job, golden_dev, oracle):
 def test_synthetic_parity(ene, oracle, P, L, nl, np_, seed):
     job = g.synth_complex(P, L, nl, np_, 2, seed=seed, ene=ene)
     dev = g.DeviceJob(job)
-    conf, pose = random_poses(job, 96, seed=seed + 100, trans=1.0)
+    conf, pose = random_poses(job, max(96, N_POSES), seed=seed + 100, trans=1.0)
     _parity(job, dev, oracle, conf, pose, "synthetic P=%d L=%d" % (P, L))
 
 
