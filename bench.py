@@ -183,6 +183,7 @@ def batched(kind, n_jobs, mc_steps):
     for d in devs:
         d.close()
     tb = time.perf_counter() - t0
+    print("[%s] batched leg: %d jobs in %.1f s" % (kind, n_jobs, tb), file=sys.stderr, flush=True)
     rep_steps = sum(h.n_rep for h in hosts) * mc_steps
     # plain path: every complex one at a time, default tiling
     t0 = time.perf_counter()
@@ -190,6 +191,9 @@ def batched(kind, n_jobs, mc_steps):
         dp = g.DeviceJob(hosts[i])
         dp.dock(mcs[i], stats=False)
         dp.close()
+        if (i + 1) % 32 == 0:
+            print("[%s] plain leg: %d jobs in %.1f s" % (kind, i + 1, time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
     tp = time.perf_counter() - t0
     # each compared job's best energies against its own plain run (same tiling)
     same = []
