@@ -82,8 +82,8 @@ struct BatchLists {
       axes.push_back(((h.xaxes & GD_EXCH_TEMP) && h.n_tmp > 1 ? GD_EXCH_TEMP : 0) |
                      ((h.xaxes & GD_EXCH_LIG) && h.n_lig > 1 ? GD_EXCH_LIG : 0) |
                      ((h.xaxes & GD_EXCH_PRT) && h.n_prt > 1 ? GD_EXCH_PRT : 0));
-      // job-local items in protein-tile-major order: waves that start together
-      // share the same protein tile (scalar-cache and L2 reuse)
+      // job-local items in the job's order (chunks by decreasing KDE work, then
+      // tile groups, then replica blocks)
       for (const GdItem &it : j->items) items.push_back({k, it.rb, it.a, it.b});
       for (int rb = 0; rb < j->h.Rp; rb += 64) blocks.push_back({k, rb});
     }

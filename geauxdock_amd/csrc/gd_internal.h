@@ -30,6 +30,7 @@ struct gd_job {
   std::vector<float> src_pspv;     // [n_types + 1][P]
   std::vector<int> src_key;        // [P] class * GD_PTAB_PT + point type
   std::vector<int> src_spread;     // [P] order inside a (class, type) group
+  std::vector<int> atom_kde;       // [L] KDE anchors of each device atom (item order)
   int laid_nP = 0;                 // tile count of the device order (0: none yet)
   hipStream_t stream = nullptr;
 };
