@@ -613,6 +613,48 @@ __device__ __forceinline__ float gd_sum_strided(const GD_AS_GLOBAL float *__rest
   return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 }
 
+// Common shapes of the reduce kernel: at most QL linear slots, one atom and one
+// MCS row per wave, at most QN tile groups and chunks.  Every load of the wave
+// is issued before the first sum, one wait in all.  Indices are clamped into
+// range and the extra values masked to zero; the sums run in slot order, so
+// the result does not depend on QL or QN.
+template <int QL, int QN>
+__device__ __forceinline__ void gd_reduce_common(
+    const GD_AS_GLOBAL float *__restrict__ part_lin, const GD_AS_GLOBAL float *__restrict__ part_hpc,
+    const GD_AS_GLOBAL float *__restrict__ part_mcs, const GD_AS_CONST GdAtom *__restrict__ atoms,
+    const GD_AS_CONST float2 *__restrict__ mcsrow, const int nS, const int nG, const int nL, const int L,
+    const int M, const int Rp, const int r, const int wi, const int q0, const int q1, float (&acc)[8]) {
+  const int ql = max(q1 - 1, q0), l = min(wi, L - 1), m = min(wi, max(M - 1, 0));
+  float vl[GD_NLIN][QL], vh[QN], vm[QN];
+#pragma unroll
+  for (int c = 0; c < GD_NLIN; ++c)
+#pragma unroll
+    for (int i = 0; i < QL; ++i) vl[c][i] = part_lin[((size_t)c * nS + min(q0 + i, ql)) * Rp + r];
+#pragma unroll
+  for (int i = 0; i < QN; ++i) vh[i] = part_hpc[((size_t)l * nG + min(i, nG - 1)) * Rp + r];
+#pragma unroll
+  for (int i = 0; i < QN; ++i) vm[i] = part_mcs[((size_t)m * nL + min(i, nL - 1)) * Rp + r];
+#pragma unroll
+  for (int c = 0; c < GD_NLIN; ++c)
+#pragma unroll
+    for (int i = 0; i < QL; ++i) acc[c] += q0 + i < q1 ? vl[c][i] : 0.0f;
+  float h = 0.0f, d = 0.0f;
+#pragma unroll
+  for (int i = 0; i < QN; ++i) {
+    h += i < nG ? vh[i] : 0.0f;
+    d += i < nL ? vm[i] : 0.0f;
+  }
+  if (wi < L) {  // hpc per atom (energy_prt_v1.C:132-133)
+    const GdAtom at = atoms[l];
+    const float t = (h - at.hpl0) / at.hpl1;
+    acc[6] += 0.5f * t * t - at.hpl2;
+  }
+  if (wi < M) {  // MCS rows (energy_mcs_v1.C:5-29)
+    const float2 row = mcsrow[m];
+    acc[7] += row.x * sqrtf(d / row.y);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Reduce kernel: GD_RK work groups (4 waves) per 64-replica block; lanes are
 // replicas.  The 4 GD_RK waves of a block split the partial slots: each takes a
@@ -644,39 +686,12 @@ __global__ __launch_bounds__(256) void gd_reduce_kernel(const GdJob *__restrict_
   for (int c = 0; c < 8; ++c) acc[c] = 0.0f;
   const int q0 = (int)(((long long)nS * wi) / W), q1 = (int)(((long long)nS * (wi + 1)) / W);
   const int nG = J.nG, nL = J.nL, L = J.L, M = J.M;
-  if (q1 - q0 <= 4 && nG <= 16 && nL <= 16 && L <= W && M <= W) {
-    // Common shape (at most 4 linear slots, one atom and one MCS row per wave):
-    // every load of the wave is issued before the first sum, one wait in all.
-    // Indices are clamped into range and the extra values masked to zero.
-    const int ql = max(q1 - 1, q0), l = min(wi, L - 1), m = min(wi, max(M - 1, 0));
-    float vl[GD_NLIN][4], vh[16], vm[16];
-#pragma unroll
-    for (int c = 0; c < GD_NLIN; ++c)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) vl[c][i] = part_lin[((size_t)c * nS + min(q0 + i, ql)) * Rp + r];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) vh[i] = part_hpc[((size_t)l * nG + min(i, nG - 1)) * Rp + r];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) vm[i] = part_mcs[((size_t)m * nL + min(i, nL - 1)) * Rp + r];
-#pragma unroll
-    for (int c = 0; c < GD_NLIN; ++c)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[c] += q0 + i < q1 ? vl[c][i] : 0.0f;
-    float h = 0.0f, d = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      h += i < nG ? vh[i] : 0.0f;
-      d += i < nL ? vm[i] : 0.0f;
-    }
-    if (wi < L) {  // hpc per atom (energy_prt_v1.C:132-133)
-      const GdAtom at = atoms[l];
-      const float t = (h - at.hpl0) / at.hpl1;
-      acc[6] += 0.5f * t * t - at.hpl2;
-    }
-    if (wi < M) {  // MCS rows (energy_mcs_v1.C:5-29)
-      const float2 row = mcsrow[m];
-      acc[7] += row.x * sqrtf(d / row.y);
-    }
+  if ((nS + W - 1) / W <= 2 && nG <= 12 && nL <= 12 && L <= W && M <= W) {
+    gd_reduce_common<2, 12>(part_lin, part_hpc, part_mcs, atoms, mcsrow, nS, nG, nL, L, M, Rp, r, wi, q0,
+                            q1, acc);
+  } else if (q1 - q0 <= 4 && nG <= 16 && nL <= 16 && L <= W && M <= W) {
+    gd_reduce_common<4, 16>(part_lin, part_hpc, part_mcs, atoms, mcsrow, nS, nG, nL, L, M, Rp, r, wi, q0,
+                            q1, acc);
   } else {
 #pragma unroll
     for (int c = 0; c < GD_NLIN; ++c)
