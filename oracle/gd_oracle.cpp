@@ -24,11 +24,12 @@
 // conformer axis, pose swaps between neighbour conformers (swap_axis below).
 //
 // Arithmetic modes:
-//   FP32 "gpu" rounding  : the squared distance is fmaf(dx,dx,fmaf(dy,dy,dz*dz))
-//                          and MCS sums run in type-sorted atom order -- one
-//                          fixed rounding sequence shared with the HIP kernels,
-//                          so the discontinuous psp cutoff decides identically
-//                          on identical coordinates;
+//   FP32 "gpu" rounding  : the squared distance is fmaf(dx,dx,fmaf(dy,dy,dz*dz)),
+//                          MCS sums run in type-sorted atom order and the DST
+//                          offset is m + (center - pocket) -- one fixed rounding
+//                          sequence shared with the HIP kernels, so the
+//                          discontinuous psp cutoff decides identically on
+//                          identical coordinates;
 //   FP32 "literal"       : the reference's own expression order (dx*dx+dy*dy+
 //                          dz*dz, MCS in file atom order); with -ffp-contract=off
 //                          it reproduces a g++ build of the reference bit for bit;
