@@ -152,18 +152,24 @@ def _sizes(rng, n, kind):
     return out
 
 
-def batched(kind, n_jobs, mc_steps):
+def batched(kind, n_jobs, mc_steps, batch_size=0):
     """Batched form (one launch sequence, persistent work-queue pair kernel,
     graph-captured segments) vs the plain one-job-per-launch path on the
     same complexes; jobs/s of each.  Both legs are timed from the host
     complexes to the results: device-job creation (packing + H2D), the run and
-    the release are inside each leg's clock, and the plain leg runs every job."""
+    the release are inside each leg's clock, and the plain leg runs every job.
+    batch_size > 0 runs the batched leg in consecutive batches of that many
+    jobs (a screen larger than one batch should hold)."""
     import geauxdock_amd as g
     from geauxdock_amd import abi
     rng = np.random.default_rng(3 if kind == "c3" else 4)
     sizes = _sizes(rng, n_jobs, kind)
-    hosts = [g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.004,
-                             ceiling_temp=0.04, seed=100 + i) for i, z in enumerate(sizes)]
+    hosts = []
+    for i, z in enumerate(sizes):
+        hosts.append(g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.004,
+                                     ceiling_temp=0.04, seed=100 + i))
+        if (i + 1) % 256 == 0:
+            print("[%s] %d complexes generated" % (kind, i + 1), file=sys.stderr, flush=True)
     mcs = [g.make_mc(mc_steps, steps_per_exchange=100, move_mode="gaussian", seed=5, job_id=i,
                      axes=AXES) for i in range(n_jobs)]
     mca = (abi.gd_mc_param * n_jobs)(*mcs)
@@ -172,18 +178,28 @@ def batched(kind, n_jobs, mc_steps):
     # the process's first device job pays the runtime's one-time set-up
     # (about 0.15 s): taken here, outside both legs
     g.DeviceJob(hosts[0]).close()
+    bs = batch_size if batch_size > 0 else n_jobs
+    t_setup = t_run = 0.0
     t0 = time.perf_counter()
-    devs = [g.DeviceJob(h) for h in hosts]
-    for d in devs:                       # shared-launch tiling: 4 atoms x 64 points
-        d.set_tiling(4, min(64, d.host.desc.prt_npoint))
-    arr = (C.c_void_p * n_jobs)(*[d.h.value for d in devs])
-    t_setup = time.perf_counter() - t0
-    g.check(g.lib().gd_batch_dock(arr, n_jobs, mca, oa, None), "gd_batch_dock")
-    t_run = time.perf_counter() - t0 - t_setup
-    for d in devs:
-        d.close()
+    for lo in range(0, n_jobs, bs):
+        hi = min(n_jobs, lo + bs)
+        ta = time.perf_counter()
+        devs = [g.DeviceJob(h) for h in hosts[lo:hi]]
+        for d in devs:                   # shared-launch tiling: 4 atoms x 64 points
+            d.set_tiling(4, min(64, d.host.desc.prt_npoint))
+        arr = (C.c_void_p * (hi - lo))(*[d.h.value for d in devs])
+        tb_ = time.perf_counter()
+        mc_lo = C.cast(C.addressof(mca) + lo * C.sizeof(abi.gd_mc_param), C.POINTER(abi.gd_mc_param))
+        out_lo = C.cast(C.addressof(oa) + lo * C.sizeof(abi.gd_replica_out), C.POINTER(abi.gd_replica_out))
+        g.check(g.lib().gd_batch_dock(arr, hi - lo, mc_lo, out_lo, None), "gd_batch_dock")
+        tc = time.perf_counter()
+        for d in devs:
+            d.close()
+        t_setup += tb_ - ta
+        t_run += tc - tb_
+        print("[%s] batched leg: %d jobs in %.1f s" % (kind, hi, time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
     tb = time.perf_counter() - t0
-    print("[%s] batched leg: %d jobs in %.1f s" % (kind, n_jobs, tb), file=sys.stderr, flush=True)
     rep_steps = sum(h.n_rep for h in hosts) * mc_steps
     # plain path: every complex one at a time, default tiling
     t0 = time.perf_counter()
@@ -207,7 +223,7 @@ def batched(kind, n_jobs, mc_steps):
             "batched_jobs_per_s": n_jobs / tb, "batched_replica_steps_per_s": rep_steps / tb,
             "plain_jobs_per_s": n_jobs / tp,
             "speedup_jobs_per_s": tp / tb, "timed": "host complexes to results, both legs",
-            "batched_setup_s": t_setup, "batched_run_s": t_run,
+            "batched_setup_s": t_setup, "batched_run_s": t_run, "batch_size": bs,
             "best_energy_bitwise_equal_to_plain": same}
 
 
@@ -338,6 +354,8 @@ def main():
                          "profiles/pair_kernel_c2_traffic.json (no GPU needed)")
     ap.add_argument("--jobs", type=int, default=32)
     ap.add_argument("--mc-steps", type=int, default=500)
+    ap.add_argument("--batch-size", type=int, default=0,
+                    help="c3/c4: jobs per batched launch sequence (0: all jobs in one)")
     args = ap.parse_args()
     if args.traffic:
         print(json.dumps(traffic_summary(*args.traffic, os.path.join(ROOT, "profiles",
@@ -358,7 +376,7 @@ def main():
         print(json.dumps(plain_c1()), flush=True)
         return
     if args.workload in ("c3", "c4"):
-        print(json.dumps(batched(args.workload, args.jobs, args.mc_steps)),
+        print(json.dumps(batched(args.workload, args.jobs, args.mc_steps, args.batch_size)),
               flush=True)
         return
     if args.workload == "c5":
