@@ -265,6 +265,9 @@ def serve(rates, n_jobs, step_scale):
             end = time.perf_counter() - t0
             for k in range(n.value):
                 lat.append(end - submit_t[rr[k].ticket])
+            if (done + n.value) // 50 > done // 50:
+                print("[c5] rate %d/s: %d of %d jobs done in %.1f s" % (lam, done + n.value, n_jobs, end),
+                      file=sys.stderr, flush=True)
             done += n.value
             if na.value == 0 and sub < n_jobs:
                 time.sleep(max(0.0, arr_t[sub] - (time.perf_counter() - t0)))
@@ -279,6 +282,9 @@ def serve(rates, n_jobs, step_scale):
         d = g.DeviceJob(hosts[i])
         d.dock(mcs[i], stats=False)
         d.close()
+        if (i + 1) % 50 == 0:
+            print("[c5] plain: %d jobs in %.1f s" % (i + 1, time.perf_counter() - t0), file=sys.stderr,
+                  flush=True)
     tp = time.perf_counter() - t0
     return {"workload": "c5-serving", "step_scale": step_scale, "per_rate": res,
             "plain_jobs_per_s": n_jobs / tp,
