@@ -38,6 +38,23 @@ AXES = ("temp", "lig", "prt")   # replica exchange across temperature x ligand x
 SEED = 2
 
 
+C2_JOB_STEPS = 10000   # MC steps of one docking job of the headline workload
+
+
+def c2_rates(n_rep, mc_steps, wall_s):
+    """(replica-steps/s, docking jobs/s) of a timed run of mc_steps MC steps of
+    the headline job; one job is the whole C2_JOB_STEPS-step run, so the timed
+    run is mc_steps / C2_JOB_STEPS of a job."""
+    return n_rep * mc_steps / wall_s, (mc_steps / float(C2_JOB_STEPS)) / wall_s
+
+
+def c2_workload(mc_steps):
+    return ("plain one-job-per-launch: P=3000 points x 11 conformers, L=40 atoms x "
+            "20 conformers, 8 temperatures 0.004-0.04 (1760 replicas), exchange every 100 "
+            "MC steps (temperature, ligand conformer, protein conformer); %d MC steps of "
+            "the %d-step job timed" % (mc_steps, C2_JOB_STEPS))
+
+
 def algorithmic_flops_per_step(job):
     """FP32 flops of one pair-kernel launch (one MC step of every replica)."""
     a = job.arrays()
@@ -50,21 +67,65 @@ def algorithmic_flops_per_step(job):
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it, with its family / model numbers
+    (the name alone can be generic under virtualisation)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                info[k] = v.strip()
+            if line.strip() == "" and info:
+                break
     except OSError:
         pass
-    return "unknown"
+    name = info.get("model name", "unknown")
+    if "cpu family" in info:
+        name += " (family %s model %s stepping %s)" % (info["cpu family"], info.get("model", "?"),
+                                                       info.get("stepping", "?"))
+    return name
+
+
+def host_cores():
+    """(logical CPUs this process may run on, physical cores among them,
+    physical cores of the whole host)."""
+    try:
+        allowed = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = list(range(os.cpu_count() or 1))
+
+    def core_of(c):
+        base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+        try:
+            return (open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip())
+        except OSError:
+            return ("?", str(c))
+    phys_allowed = len({core_of(c) for c in allowed})
+    n_all = os.cpu_count() or len(allowed)
+    phys_all = len({core_of(c) for c in range(n_all)})
+    return len(allowed), phys_allowed, phys_all
 
 
 def cpu_threads():
-    """Host threads for the CPU leg: the job's CPU share (OMP_NUM_THREADS when
-    set, at most 16), one per core, threads bound (OMP_PROC_BIND)."""
+    """Host threads for the CPU leg: one per physical core this process may
+    use, at most 16 (the CPU share of a one-GPU job; OMP_NUM_THREADS, when set,
+    caps it), threads bound (OMP_PROC_BIND)."""
     os.environ.setdefault("OMP_PROC_BIND", "true")
-    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    _, phys, _ = host_cores()
+    n = phys
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if env > 0:
+        n = min(n, env)
     return max(1, min(16, n))
+
+
+def cpu_fields(threads):
+    """The CPU leg's hardware description: threads used, cores, model."""
+    logical, phys, phys_all = host_cores()
+    return {"cores": threads, "cpu_model": cpu_model(), "affinity_logical_cpus": logical,
+            "affinity_physical_cores": phys, "host_physical_cores": phys_all,
+            "threads_note": "one OpenMP thread per physical core, at most 16 (a one-GPU job's CPU share)"}
 
 
 def cpu_baseline(job, mc_template, budget_s=15.0):
@@ -84,11 +145,12 @@ def cpu_baseline(job, mc_template, budget_s=15.0):
     _, _, ms = orc.dock(job, mc, threads=threads)
     # ms covers the initial pass + `steps` steps
     rs = job.n_rep * (steps + 1) / (ms / 1000.0)
-    return {"value": rs, "unit": "replica-steps/s", "cores": threads, "kind": "port",
-            "sample": "all %d replicas x %d MC steps (+ initial pass) of the same complex, "
-                      "OpenMP over replicas; the reference backend's fixed capacities (2048 protein "
-                      "points) cannot hold this complex, so its restatement is timed" % (job.n_rep, steps),
-            "cpu_model": cpu_model()}
+    out = {"value": rs, "unit": "replica-steps/s", "kind": "port",
+           "sample": "all %d replicas x %d MC steps (+ initial pass) of the same complex, "
+                     "OpenMP over replicas; the reference backend's fixed capacities (2048 protein "
+                     "points) cannot hold this complex, so its restatement is timed" % (job.n_rep, steps)}
+    out.update(cpu_fields(threads))
+    return out
 
 
 def plain_c1():
@@ -391,6 +453,8 @@ def main():
     name = C.create_string_buffer(128)
     ncu, clk, hbm = C.c_int(), C.c_int(), C.c_size_t()
     g.lib().gd_device_info(name, 128, C.byref(ncu), C.byref(clk), C.byref(hbm))
+    sclk = C.c_double()
+    g.check(g.lib().gd_measure_sclk(200, C.byref(sclk)), "gd_measure_sclk")
 
     job = g.synth_complex(P, L, NLIG, NPRT, NTMP, floor_temp=0.004, ceiling_temp=0.04, seed=SEED)
     dev = g.DeviceJob(job)
@@ -405,8 +469,10 @@ def main():
     t0 = time.perf_counter()
     out, st = dev.dock(mc, stats=True)     # returns after the device is idle
     wall = time.perf_counter() - t0
-    rep_steps = job.n_rep * steps
-    value = rep_steps / wall
+    # the engine clock under an all-CU FMA load, right after the timed region
+    sclk_after = C.c_double()
+    g.check(g.lib().gd_measure_sclk(200, C.byref(sclk_after)), "gd_measure_sclk")
+    value, jobs_per_s = c2_rates(job.n_rep, steps, wall)
     pair_ms = st.ms_pair_kernel / max(1, st.n_pair_launch)
     achieved = flops_step / (pair_ms / 1000.0) / 1e12
     traffic = None
@@ -427,9 +493,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded generator: random-walk protein, random-tree ligand)",
         "config": {
-            "workload": "plain one-job-per-launch: P=3000 points x 11 conformers, L=40 atoms x "
-                        "20 conformers, 8 temperatures 0.004-0.04 (1760 replicas), 10000 MC steps, "
-                        "exchange every 100 (temperature, ligand conformer, protein conformer)",
+            "workload": c2_workload(steps),
             "replicas": job.n_rep, "mc_steps": steps, "prt_npoint": P, "lig_natom": L,
             "kde_npoint": job.desc.kde_npoint, "mcs_nrow": job.desc.mcs_nrow,
             "step_unit": "100 MC steps + 1 exchange point",
@@ -438,10 +502,15 @@ def main():
                                "ligand_conformer": int(out.n_swap_lig.sum()) // 2,
                                "protein_conformer": int(out.n_swap_prt.sum()) // 2},
             "tiling": {"atoms_per_lane": dev.tiling()[0], "tile_points": dev.tiling()[1]},
-            "device": name.value.decode(), "cu": ncu.value, "sclk_mhz": clk.value / 1000.0,
+            "device": name.value.decode(), "cu": ncu.value, "hbm_bytes": hbm.value,
+            "sclk_mhz_nominal": clk.value / 1000.0,
+            "sclk_mhz_measured": [sclk.value, sclk_after.value],
+            "sclk_note": "engine clock under an all-CU FMA load (shader-clock counter against the "
+                         "constant-rate counter), before and after the timed region",
             "peak_from_device_tflops": ncu.value * 64 * 2 * 2 * clk.value * 1e3 / 1e12,
+            "peak_from_measured_clock_tflops": ncu.value * 64 * 2 * 2 * sclk_after.value * 1e6 / 1e12,
         },
-        "jobs_per_s": 1.0 / wall * (10000.0 / steps),
+        "jobs_per_s": jobs_per_s,
         "pair_evals_per_s": prt_pairs * (steps + 1) / wall,
         "roofline": {
             "bound": "valu",

@@ -100,6 +100,12 @@ class gd_job_result(C.Structure):
     ]
 
 
+# gd_record_fn: (user, dump, step_begin, n_rep, cap, rec, count)
+RECORD_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
+                        C.POINTER(C.c_int))
+GD_STREAM_BLOCKING = 1
+REC_W = 17   # floats per accept record: step, movematrix[6], energy[10]
+
 # Every extern "C" entry point of include/geauxdock_amd.h, with its ctypes
 # prototype (restype, argtypes).  Tests check the library exports all of them.
 _vp = C.c_void_p
@@ -117,6 +123,11 @@ PROTOTYPES = {
     "gd_dock": (C.c_int, [_vp, C.POINTER(gd_mc_param), C.POINTER(gd_replica_out),
                           C.POINTER(gd_stats)]),
     "gd_dock_records": (C.c_int, [_vp, C.c_int, _fp, _ip]),
+    "gd_dock_stream": (C.c_int, [_vp, C.POINTER(gd_mc_param), _vp, _vp, C.c_int,
+                                 C.POINTER(gd_replica_out), C.POINTER(gd_stats)]),
+    "gd_record_csv_open": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "gd_record_csv_write": (None, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _ip]),
+    "gd_record_csv_close": (C.c_int, [_vp]),
     "gd_job_replicas": (C.c_int, [_vp, C.POINTER(gd_replica_out)]),
     "gd_batch_dock": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(gd_mc_param),
                                 C.POINTER(gd_replica_out), C.POINTER(gd_stats)]),
@@ -145,15 +156,19 @@ PROTOTYPES = {
     "gd_host_job_desc": (C.c_int, [_vp, C.POINTER(gd_job_desc)]),
     "gd_host_job_free": (C.c_int, [_vp]),
     "gd_kabsch_rmsd": (_f, [C.c_int, _fp, _fp]),
+    "gd_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int,
+                         C.POINTER(C.c_uint32)]),
+    "gd_move_draws": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int, _fp, C.c_int, C.c_uint32,
+                                C.c_int, C.c_int, _fp]),
+    "gd_move_incr_words": (C.c_int, [C.c_int, _fp, C.c_int, C.POINTER(C.c_uint32), _fp]),
+    "gd_measure_sclk": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
+    "gd_far_r2": (C.c_float, [C.c_float, C.c_float]),
+    "gd_near_r2": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
 }
 
 # Helpers exported beside the header API (not part of include/geauxdock_amd.h).
 EXTRA_PROTOTYPES = {
     "gd_host_job_diag": (C.c_int, [_vp, _ip, _fp, C.c_int, _ip, C.c_int]),
-    "gd_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int,
-                         C.POINTER(C.c_uint32)]),
-    "gd_far_r2": (C.c_float, [C.c_float, C.c_float]),
-    "gd_near_r2": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
 }
 
 

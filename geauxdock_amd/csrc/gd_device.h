@@ -59,7 +59,7 @@ struct GdJob {
   const float4 *pattr;    // [P] (ele, hpp, flags-as-int, 0)
   const float *ptab;      // [n_types + 1][GD_PTAB_PT][GD_PTAB_W] (last block: padding atoms):
                           // 1/p1a, -p2a/p1a, negated PMF exponent B and C,
-                          // pmf1, psp r^2 threshold (sign bit: H-bond active), H-bond c, -hdb0 c,
+                          // pmf1, psp r^2 threshold (sign bit: H-bond active), H-bond hdb1, hdb0,
                           // far-field r^2 threshold
   const float *pspv;      // [n_types][P] psp value of class-2 points
   const float4 *kde;      // [K] sorted by type
@@ -82,8 +82,9 @@ struct GdJob {
   float *part_hpc;        // [L][nG][Rp]
   float *part_mcs;        // [M][nL][Rp]
   float *fin_part;        // [Rp/64][GD_RK][8][64]: per-slice sums of the reduce kernel
-  float *rec;             // [Rp][rec_cap][GD_REC_W] accept records (optional)
-  int *rec_n;             // [Rp]
+  float *rec;             // [2][Rp][rec_cap][GD_REC_W] accept records (optional), a
+                          // two-half ring: dump d fills half d mod 2
+  int *rec_n;             // [2][Rp] entries used per half
   int rec_cap;
   float *ev_terms;        // [Rp][10] eval outputs
   float *ev_raw;          // [Rp][9]
@@ -116,6 +117,10 @@ struct GdItem {           // one 4-wave work group of the pair kernel
 struct GdBlock {          // one workgroup of the finish / transform kernels
   int job;
   int rb;
+};
+
+struct GdScale6 {         // move scales passed by value
+  float v[6];
 };
 
 struct GdExch {           // one workgroup of the exchange kernel

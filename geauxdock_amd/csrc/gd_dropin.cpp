@@ -203,18 +203,61 @@ void InitCurand(gd_rng_state **s) {
   s[0]->job_id = 0;
 }
 
+// Copies one dump's accept records into the caller's Record array: what the
+// reference's per-dump cudaMemcpy into rh does (backend_gpu.cu:110-116), so rh
+// holds the last dump's records when Dock returns.
+struct DumpToRecords {
+  Record *rh;
+  const Complex *ch;
+  int rep_begin, rep_end;
+};
+
+static void dump_to_records(void *user, int dump, int step_begin, int n_rep, int cap, const float *rec,
+                            const int *count) {
+  (void)dump;
+  (void)step_begin;
+  const DumpToRecords *d = (const DumpToRecords *)user;
+  for (int r = d->rep_begin; r <= d->rep_end && r < n_rep; ++r) {
+    Record &R = d->rh[r];
+    const int n = std::min(std::min(count[r], cap), GD_REF_STEPS_PER_DUMP);
+    R.next_entry = n;
+    const ReplicaMC &base = d->ch->replica[r];
+    for (int k = 0; k < n; ++k) {
+      const float *o = rec + ((size_t)r * cap + k) * 17;
+      ReplicaMC &q = R.replica[k];
+      q = base;
+      q.step = (int)o[0];
+      for (int i = 0; i < 6; ++i) q.movematrix[i] = o[1 + i];
+      for (int i = 0; i < 10; ++i) q.energy[i] = o[7 + i];
+      q.is_accept = 1;
+    }
+  }
+}
+
+// Temperature-exchange period of the drop-in Dock.  The reference GPU Dock
+// never exchanges (backend_gpu.cu:60-133), and its frontend never writes
+// McPara.steps_per_exchange (server1.C allocates McPara uninitialised;
+// ParseArguments, util.C:53-190, does not set it), so the field is not read:
+// exchange is off unless GD_EXCHANGE_STEPS names a period.
+static int dropin_exchange_period(int steps_total) {
+  const char *env = std::getenv("GD_EXCHANGE_STEPS");
+  if (!env) return 0;
+  const long v = std::strtol(env, nullptr, 10);
+  return (v > 0 && v <= steps_total) ? (int)v : 0;
+}
+
 void Dock(Complex *ch, Record *rh, Complex **cd, Record **rd, ParaT **pt, gd_rng_state **s) {
   (void)rd;
   DropinCtx *x = ctx_of(cd);
   const int steps_total = ch->mcpara.steps_total;
   const int steps_per_dump = ch->mcpara.steps_per_dump > 0 ? ch->mcpara.steps_per_dump : 1;
+  if (steps_per_dump > GD_REF_STEPS_PER_DUMP) die(GD_ERR_SIZE, "Dock: steps_per_dump > STEPS_PER_DUMP");
   const int n_rep = ch->size.n_rep;
   gd_mc_param mc;
   std::memset(&mc, 0, sizeof mc);
   mc.steps_total = steps_total;
   mc.steps_per_dump = steps_per_dump;
-  const int spe = ch->mcpara.steps_per_exchange;
-  mc.steps_per_exchange = (spe > 0 && spe <= steps_total) ? spe : 0;
+  mc.steps_per_exchange = dropin_exchange_period(steps_total);
   mc.move_mode = GD_MOVE_REF_UNIFORM;
   for (int i = 0; i < 6; ++i) mc.move_scale[i] = ch->mcpara.move_scale[i];
   mc.seed = s[0]->seed;
@@ -230,32 +273,14 @@ void Dock(Complex *ch, Record *rh, Complex **cd, Record **rd, ParaT **pt, gd_rng
   out.idx_tmp = tmp.data();
   out.is_accept = acc.data();
   gd_stats stats;
+  for (int r = pt[0]->rep_begin; r <= pt[0]->rep_end; ++r) rh[r].next_entry = 0;
+  DumpToRecords sink = {rh, ch, pt[0]->rep_begin, pt[0]->rep_end};
   const double t0 = now_ms();
-  gd_status st = gd_dock(x->job, &mc, &out, &stats);
+  // every dump's records copied into rh after the dump, blocking, as the reference does
+  gd_status st = gd_dock_stream(x->job, &mc, dump_to_records, &sink, GD_STREAM_BLOCKING, &out, &stats);
   if (st != GD_OK) die(st, "Dock");
   const double t_mc = now_ms() - t0;
-  // the last dump's accept records -> the caller's Record array
-  const int cap = GD_REF_STEPS_PER_DUMP;
-  std::vector<float> rec((size_t)n_rep * cap * 17);
-  std::vector<int> cnt(n_rep);
-  if (steps_total > 0) {
-    st = gd_dock_records(x->job, cap, rec.data(), cnt.data());
-    if (st != GD_OK) die(st, "Dock records");
-  }
   for (int r = pt[0]->rep_begin; r <= pt[0]->rep_end; ++r) {
-    Record &R = rh[r];
-    const int n = steps_total > 0 ? std::min(cnt[r], cap) : 0;
-    R.next_entry = n;
-    const ReplicaMC &base = ch->replica[r];
-    for (int k = 0; k < n; ++k) {
-      const float *o = &rec[((size_t)r * cap + k) * 17];
-      ReplicaMC &q = R.replica[k];
-      q = base;
-      q.step = (int)o[0];
-      for (int i = 0; i < 6; ++i) q.movematrix[i] = o[1 + i];
-      for (int i = 0; i < 10; ++i) q.energy[i] = o[7 + i];
-      q.is_accept = 1;
-    }
     ReplicaMC &rep = ch->replica[r];
     for (int i = 0; i < 6; ++i) rep.movematrix[i] = mm[(size_t)r * 6 + i];
     for (int i = 0; i < 10; ++i) rep.energy[i] = en[(size_t)r * 10 + i];

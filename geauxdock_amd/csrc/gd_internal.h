@@ -17,8 +17,12 @@ struct gd_job {
   void *d_base = nullptr;    // tables + replica state
   void *d_part = nullptr;    // partial sums (tiling-dependent)
   void *d_lists = nullptr;   // work items + blocks
-  void *d_rec = nullptr;     // accept records
+  void *d_rec = nullptr;     // accept records: a two-half ring, one half per dump
   size_t rec_bytes = 0;
+  void *h_rec = nullptr;     // pinned host copies of both halves (record streaming)
+  size_t h_rec_bytes = 0;
+  hipStream_t copy_stream = nullptr;  // D2H of finished dumps beside the MC stream
+  int last_half = -1;        // ring half of the last completed dump (-1: none)
   size_t bytes = 0;          // d_base size
   std::vector<GdItem> items; // job-local work items (job index 0)
   GdItem *d_items = nullptr;

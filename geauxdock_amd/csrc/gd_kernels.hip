@@ -46,8 +46,8 @@ __device__ __forceinline__ float gd_exp2(float x) { return __builtin_amdgcn_exp2
 #define GD_HPB (4.5f / 81.0f / 81.0f)
 #define GD_HPC (2.5f / 81.0f / 81.0f / 81.0f)
 #define GD_HPD (0.5f / 81.0f / 81.0f / 81.0f / 81.0f)
-// 1 / sqrt(log2(e) / 2) = sqrt(2 ln 2): H-bond sums carry the factor sqrt(log2(e) / 2)
-#define GD_HDB_RC 1.17741002251547469f
+// log2(e) / 2 in FP64: the H-bond exponent is formed in double precision
+#define GD_HALF_LOG2E_D 0.72134752044448170368
 
 // ---------------------------------------------------------------------------
 // Address-space helpers: tables shared by all lanes are read through the
@@ -76,6 +76,24 @@ __device__ __forceinline__ const GD_AS_GLOBAL T *gd_g(const T *p) {
 }
 #endif
 
+// FP32 -> FP64 conversions that the compiler may not move (see the H-bond
+// term); _s takes a wave-uniform value held in a scalar register.
+__device__ __forceinline__ double gd_f2d(float x) {
+  double d;
+  asm volatile("v_cvt_f64_f32 %0, %1" : "=v"(d) : "v"(x));
+  return d;
+}
+__device__ __forceinline__ double gd_f2d_s(float x) {
+  double d;
+  asm volatile("v_cvt_f64_f32 %0, %1" : "=v"(d) : "s"(x));
+  return d;
+}
+// r2 += (a - b)^2 with the difference exact in FP64
+__device__ __forceinline__ void gd_hdb_axis(float a, float b, double &r2) {
+  const double d = gd_f2d(a) - gd_f2d(b);
+  r2 = __builtin_fma(d, d, r2);
+}
+
 // ---------------------------------------------------------------------------
 // Pair kernel.  LC ligand atoms per wave, all LC coordinates in registers.
 
@@ -101,7 +119,7 @@ __device__ __forceinline__ void gd_pair_one(
     const float pmf_a, const float lxj, const float lyj, const float lzj, float &elej, float &hpcj,
     float &evdw, float &epmf, float &epsp, float &ehdb) {
     const float w9 = rw[0], nq = rw[1], pB = rw[2], pC = rw[3];
-    const float pmf1 = rw[4], pthr = rw[5], hdb1c = rw[6], nhdb0c = rw[7], r2far = rw[8];
+    const float pmf1 = rw[4], pthr = rw[5], hdb1 = rw[6], hdb0 = rw[7], r2far = rw[8];
     // the sign bit of the psp threshold flags an active H-bond pair; the asm
     // keeps the test on the scalar unit (as a float compare it became a VALU op)
     int hflag;
@@ -176,13 +194,31 @@ __device__ __forceinline__ void gd_pair_one(
           epsp += pspv[(size_t)prow * P + p];
         }
       }
-      // hydrogen bond (:116-121) as c exp2(-(r c - hdb0 c)^2), c = hdb1 sqrt(log2(e)/2),
-      // summed without the 1/sqrt(log2(e)/2) factor; pairs with hdb0 <= 0.1 skip
-      // the term on a wave-uniform branch
+      // hydrogen bond (:116-121), hdb1 exp(-h^2 / 2) with h = (r - hdb0) hdb1,
+      // as hdb1 2^-q, q = log2(e)/2 h^2; pairs with hdb0 <= 0.1 skip the term
+      // on a wave-uniform branch.  In the tail of the Gaussian the relative
+      // error of a contribution is about q' dr, and FP32 rounding of r (and of
+      // q itself, up to ~150) would put it near 1e-5; so q is formed in FP64:
+      // exact coordinate differences, r from the FP32 r by one Newton step,
+      // then 2^-q = 2^-floor(q) * 2^-frac(q) with only frac(q) in [0, 1)
+      // rounded to FP32.  Beyond the row's r^2 threshold q >= 160, where the
+      // FP32 result is exactly 0 (far waves skip it).
       if (hflag < 0) {
         asm volatile("");
-        const float h = __builtin_fmaf(rr, hdb1c, nhdb0c);
-        ehdb = __builtin_fmaf(hdb1c, gd_exp2(-(h * h)), ehdb);
+        // the conversions are volatile asm so that they stay here: hoisted out
+        // of the point loop, the converted ligand coordinates would hold six
+        // VGPRs per atom; r^2 is accumulated one axis at a time
+        double r2d = 0.0;
+        gd_hdb_axis(lzj, c.z, r2d);
+        gd_hdb_axis(lyj, c.y, r2d);
+        gd_hdb_axis(lxj, c.x, r2d);
+        const double r0 = gd_f2d(rr);
+        const double rd = __builtin_fma(__builtin_fma(-r0, r0, r2d), gd_f2d(0.5f * ri), r0);
+        const double hd = (rd - gd_f2d_s(hdb0)) * gd_f2d_s(hdb1);
+        const double q = (hd * hd) * GD_HALF_LOG2E_D;
+        const float fq = (float)__builtin_amdgcn_fract(q);
+        const int nq = (int)q;
+        ehdb = __builtin_fmaf(hdb1, __builtin_ldexpf(gd_exp2(-fq), -nq), ehdb);
       }
     }
     elej = __builtin_fmaf(pele, g, elej);
@@ -393,7 +429,6 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   const float inv_el1 = k.inv_el1;
   float evdw = 0.0f, epmf = 0.0f, epsp = 0.0f, ehdb = 0.0f;
   gd_pair_points<LC>(J, b, pc, k, lx, ly, lz, trow, ele, hpc, evdw, epmf, epsp, ehdb);
-  ehdb *= GD_HDB_RC;
 
   // KDE: this tile's share of every chunk atom's anchor bin, a contiguous range
   // [f(b), f(b+1)) with f(b) = floor(n b / nP) evaluated in FP32: f is
@@ -517,7 +552,11 @@ template __global__ void gd_pair_kernel_wq<16>(const GdJob *, const GdItem *, in
 // ---------------------------------------------------------------------------
 // Ligand transform of one replica's proposal (kernel_cpp_montecarlo_soa.C:
 // 256-293): waves of the block split the atoms, lanes are replicas.
+// Every product of the rotation is rounded on its own (no contraction), and
+// the transform below is one explicit FMA chain, so that every kernel that
+// transforms a pose (transform, finish, swap) gives the same bits for it.
 __device__ __forceinline__ void gd_rotation(const float m[6], float R[9]) {
+#pragma clang fp contract(off)
   float s1, c1, s2, c2, s3, c3;
   sincosf(m[3], &s1, &c1);
   sincosf(m[4], &s2, &c2);
@@ -539,17 +578,22 @@ __device__ __forceinline__ void gd_rotation(const float m[6], float R[9]) {
 // load followed by a full wait.
 typedef const GD_AS_CONST GdJob GdJobC;
 
+// One atom of the transform x = R a + t (kernel_cpp_montecarlo_soa.C:274-293),
+// the conformer-relative products summed first and the lab-frame offset t
+// added last, into the [L][3][Rp] coordinate layout.
+__device__ __forceinline__ void gd_put_atom(GD_AS_GLOBAL float *co, int Rp, int r, int l,
+                                            const float R[9], const float t[3], const float4 a) {
+  co[(size_t)(3 * l + 0) * Rp + r] = __builtin_fmaf(R[0], a.x, __builtin_fmaf(R[1], a.y, R[2] * a.z)) + t[0];
+  co[(size_t)(3 * l + 1) * Rp + r] = __builtin_fmaf(R[3], a.x, __builtin_fmaf(R[4], a.y, R[5] * a.z)) + t[1];
+  co[(size_t)(3 * l + 2) * Rp + r] = __builtin_fmaf(R[6], a.x, __builtin_fmaf(R[7], a.y, R[8] * a.z)) + t[2];
+}
+
 __device__ __forceinline__ void gd_transform_atoms(GdJobC &J, int r, const float R[9],
                                                    const float t[3], int wave, int nwave) {
   const int L = J.L, Rp = J.Rp;
   const GD_AS_GLOBAL float4 *lc = gd_g(J.lig0) + (size_t)gd_g(J.ligi)[r] * L;
   GD_AS_GLOBAL float *co = gd_g(J.coords);
-  for (int l = wave; l < L; l += nwave) {
-    const float4 a = lc[l];
-    co[(size_t)(3 * l + 0) * Rp + r] = R[0] * a.x + R[1] * a.y + R[2] * a.z + t[0];
-    co[(size_t)(3 * l + 1) * Rp + r] = R[3] * a.x + R[4] * a.y + R[5] * a.z + t[1];
-    co[(size_t)(3 * l + 2) * Rp + r] = R[6] * a.x + R[7] * a.y + R[8] * a.z + t[2];
-  }
+  for (int l = wave; l < L; l += nwave) gd_put_atom(co, Rp, r, l, R, t, lc[l]);
 }
 
 // Transform the current proposal (prop) of every replica in the block.
@@ -577,21 +621,27 @@ __device__ __forceinline__ void gd_move(GdJobC &J, int r, uint32_t step, float i
                                         float *u_acc) {
   uint32_t w[8];
   gd_step_draws(((uint64_t)J.seed_hi << 32) | J.seed_lo, J.job_id, (uint32_t)r, step, w);
-  if (J.move_mode == 1) {
+  float scale[6];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float u1 = gd_u01(w[2 * k]), u2 = gd_u01(w[2 * k + 1]);
-      const float rad = sqrtf(-2.0f * logf(1.0f - u1));
-      float sn, cs;
-      sincosf(6.28318530717958647692f * u2, &sn, &cs);
-      inc[2 * k] = J.move_scale[2 * k] * (rad * cs);
-      inc[2 * k + 1] = J.move_scale[2 * k + 1] * (rad * sn);
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < 6; ++i) inc[i] = J.move_scale[i] * gd_u01(w[i]);
-  }
+  for (int i = 0; i < 6; ++i) scale[i] = J.move_scale[i];
+  gd_move_incr(J.move_mode, scale, w, inc);
   *u_acc = gd_u01(w[6]);
+}
+
+// Increments of replica-steps (replica r of n_rep, step step0 + s), one per
+// thread: the draws gd_move makes, exposed for the statistical tests.
+__global__ __launch_bounds__(256) void gd_move_draws_kernel(uint64_t seed, uint32_t job, int mode,
+                                                            GdScale6 scale, int n_rep,
+                                                            uint32_t step0, int n,
+                                                            float *__restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t w[8];
+  gd_step_draws(seed, job, (uint32_t)(i % n_rep), step0 + (uint32_t)(i / n_rep), w);
+  float inc[6];
+  gd_move_incr(mode, scale.v, w, inc);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) out[(size_t)i * 6 + k] = inc[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -871,11 +921,15 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
     } else {
       GD_AS_GLOBAL int *acc_r = gd_g(J.acc);
       // record the pre-move state if the previous step was accepted (:224-231)
+      // into the ring half of this step's dump (dump d = step / rec_cap fills
+      // half d mod 2 while the host copies out the other)
       if (mode == GD_MODE_STEP && J.record && live && acc_r[r]) {
-        GD_AS_GLOBAL int *rec_n = gd_g(J.rec_n);
+        const int half = (step / J.rec_cap) & 1;
+        GD_AS_GLOBAL int *rec_n = gd_g(J.rec_n) + (size_t)half * Rp;
         const int n = rec_n[r];
         if (n < J.rec_cap) {
-          GD_AS_GLOBAL float *o = gd_g(J.rec) + ((size_t)r * J.rec_cap + n) * GD_REC_W;
+          GD_AS_GLOBAL float *o =
+              gd_g(J.rec) + (((size_t)half * Rp + r) * J.rec_cap + n) * GD_REC_W;
           o[0] = (float)step;
           for (int i = 0; i < 6; ++i) o[1 + i] = mm[(size_t)i * Rp + r];
           for (int i = 0; i < 10; ++i) o[7 + i] = en[(size_t)i * Rp + r];
@@ -914,11 +968,7 @@ __global__ __launch_bounds__(1024) void gd_finish_kernel(const GdJob *__restrict
   t[2] = xf[c][11][lane];
   // the transform of gd_transform_atoms, the first atoms from the prefetch
   GD_AS_GLOBAL float *co = gd_g(J.coords);
-  auto put = [&](int l, const float4 a) {
-    co[(size_t)(3 * l + 0) * Rp + r] = R[0] * a.x + R[1] * a.y + R[2] * a.z + t[0];
-    co[(size_t)(3 * l + 1) * Rp + r] = R[3] * a.x + R[4] * a.y + R[5] * a.z + t[1];
-    co[(size_t)(3 * l + 2) * Rp + r] = R[6] * a.x + R[7] * a.y + R[8] * a.z + t[2];
-  };
+  auto put = [&](int l, const float4 a) { gd_put_atom(co, Rp, r, l, R, t, a); };
 #pragma unroll
   for (int q = 0; q < GD_FPA; ++q)
     if (wave + q * GD_FW < L) put(wave + q * GD_FW, pa[q]);
@@ -1096,4 +1146,29 @@ __global__ __launch_bounds__(256) void gd_swap_accept_kernel(const GdJob *__rest
   t[1] = xf[10][lane];
   t[2] = xf[11][lane];
   gd_transform_atoms(J, r, R, t, wave, 4);
+}
+
+// ---------------------------------------------------------------------------
+// Engine clock: one wave per SIMD of every CU spins on dependent FMAs for about
+// `ticks` periods of the constant-rate counter (s_memrealtime) while reading
+// the shader-clock counter (s_memtime); block 0's first wave records both
+// deltas.  Every wave leaves once its own constant-rate delta passes `ticks`,
+// or after 2^24 iterations (a few seconds) whatever the counters read.
+__global__ __launch_bounds__(64) void gd_clock_kernel(long long ticks, float seed,
+                                                       long long *__restrict__ out,
+                                                       float *__restrict__ sink) {
+  const long long w0 = wall_clock64(), c0 = clock64();
+  float a = seed + (float)threadIdx.x, b = 1.0000001f;
+  long long w1 = w0;
+  for (int it = 0; it < (1 << 24) && w1 - w0 < ticks; ++it) {
+#pragma unroll
+    for (int i = 0; i < 64; ++i) a = __builtin_fmaf(a, b, 1e-7f);
+    w1 = wall_clock64();
+  }
+  const long long c1 = clock64();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out[0] = c1 - c0;
+    out[1] = w1 - w0;
+  }
+  if (a == 12345.0f) sink[0] = a;  // keeps the FMA chain
 }

@@ -16,6 +16,7 @@
 //   slot 3  -> ligand-conformer swap draw, slot 4 -> protein-conformer swap draw
 //              (counter word 0 = round, word 1 = lower-conformer replica)
 #pragma once
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -99,4 +100,31 @@ GD_HD float gd_swap_u01(uint64_t seed, uint32_t job, uint32_t replica, uint32_t 
   const gd_u32x4 a =
       gd_philox4x32(round, replica, job, 3u + axis, (uint32_t)seed, (uint32_t)(seed >> 32));
   return gd_u01(a.v[0]);
+}
+
+// Move increments of one replica-step from its draws w[0..5] (the reference
+// move is scale * U[0,1), kernel_cpp_montecarlo_soa.C:242-250; mode 1 draws
+// scale * N(0,1) instead).  Gaussian pairs by Box-Muller: with u1, u2 uniform
+// on [0,1), r = sqrt(-2 ln(1 - u1)) (1 - u1 is in (0, 1], so r is finite) and
+// the pair (r cos 2 pi u2, r sin 2 pi u2) is two independent N(0,1) draws.
+// Compiled for the host (libm) and for the device (the device math library);
+// the two agree to a few ulps, not bit for bit.
+GD_HD void gd_move_incr(int mode, const float scale[6], const uint32_t w[8], float inc[6]) {
+  if (mode == 1) {
+    for (int k = 0; k < 3; ++k) {
+      const float u1 = gd_u01(w[2 * k]), u2 = gd_u01(w[2 * k + 1]);
+      const float rad = sqrtf(-2.0f * logf(1.0f - u1));
+      float sn, cs;
+#if defined(__HIP_DEVICE_COMPILE__)
+      sincosf(6.28318530717958647692f * u2, &sn, &cs);
+#else
+      sn = sinf(6.28318530717958647692f * u2);
+      cs = cosf(6.28318530717958647692f * u2);
+#endif
+      inc[2 * k] = scale[2 * k] * (rad * cs);
+      inc[2 * k + 1] = scale[2 * k + 1] * (rad * sn);
+    }
+  } else {
+    for (int i = 0; i < 6; ++i) inc[i] = scale[i] * gd_u01(w[i]);
+  }
 }

@@ -33,6 +33,8 @@ __global__ void gd_transform_kernel(const GdJob *, const GdBlock *, int *);
 __global__ void gd_reduce_kernel(const GdJob *, const GdBlock *);
 __global__ void gd_finish_kernel(const GdJob *, const GdBlock *, int, int, const int *, int *);
 __global__ void gd_exchange_kernel(const GdJob *, const GdExch *);
+__global__ void gd_move_draws_kernel(uint64_t, uint32_t, int, GdScale6, int, uint32_t, int, float *);
+__global__ void gd_clock_kernel(long long, float, long long *, float *);
 __global__ void gd_swap_propose_kernel(const GdJob *, const GdBlock *, int, const int *);
 __global__ void gd_swap_accept_kernel(const GdJob *, const GdBlock *, int, const int *, int,
                                       const int *);
@@ -165,8 +167,8 @@ float far_r2_threshold(double pmf0, double el1) {
 // raised to cover the terms that branch also holds, so that a wave whose lanes
 // are all beyond it skips exact zeros only:
 //   hydrophobic, r^2 <= 81; psp, r^2 <= psp_thr (negative: no psp term);
-//   H-bond (hb), c exp2(-h^2) with h = (r - hdb0) c: exp2 is exactly 0 in FP32
-//   for h^2 >= 150, and h^2 >= 160 from r = hdb0 + sqrt(160) / c on.
+//   H-bond (hb), hdb1 2^-q with q = ((r - hdb0) c)^2, c = hdb1 sqrt(log2(e)/2):
+//   2^-q is exactly 0 in FP32 for q >= 151, and q >= 160 from r = hdb0 + sqrt(160) / c on.
 static float near_r2_threshold(double pmf0, double el1, float psp_thr, bool hb, double hdb0,
                                double c) {
   double r2n = far_r2_threshold(pmf0, el1);
@@ -385,11 +387,12 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
       const float thr = sqrt_threshold(e->pmf0[lt][pt]);
       const bool hb = e->hdb0[lt][pt] > 0.1f;
       o[5] = std::copysign(thr >= 0.0f ? thr : std::numeric_limits<float>::quiet_NaN(), hb ? -1.0f : 1.0f);
-      // H-bond: hdb1 exp(-(r - hdb0)^2 hdb1^2 / 2) = (c / k) exp2(-(r c - hdb0 c)^2),
-      // k = sqrt(log2(e)/2), c = hdb1 k; rows with hdb0 <= 0.1 are off (c = 0)
+      // H-bond: hdb1 exp(-(r - hdb0)^2 hdb1^2 / 2), the raw parameters (the
+      // kernel forms the exponent in FP64); rows with hdb0 <= 0.1 are off.  The
+      // near threshold covers q = ((r - hdb0) c)^2 < 160, c = hdb1 sqrt(log2(e)/2)
       const double c = (double)e->hdb1[lt][pt] * hdb_c;
-      o[6] = hb ? (float)c : 0.0f;
-      o[7] = hb ? (float)(-(double)e->hdb0[lt][pt] * c) : 0.0f;
+      o[6] = hb ? e->hdb1[lt][pt] : 0.0f;
+      o[7] = hb ? e->hdb0[lt][pt] : 0.0f;
       o[8] = near_r2_threshold(pmf0, e->el1, thr, hb, e->hdb0[lt][pt], hb ? c : 0.0);
     }
   }
@@ -553,7 +556,7 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
   a.add(&g.best, (size_t)Rp);
   a.add(&g.own, (size_t)nl * np * nt);
   a.add(&g.coords, (size_t)3 * L * Rp);
-  a.add(&g.rec_n, (size_t)Rp);
+  a.add(&g.rec_n, (size_t)2 * Rp);
   a.add(&g.ev_terms, (size_t)10 * Rp);
   a.add(&g.ev_raw, (size_t)9 * Rp);
   a.add(&g.wq, 64);
@@ -636,6 +639,7 @@ gd_status reset_replicas(gd_job *j) {
   GD_UP(g.nswp, zero.data(), Rp * sizeof(int));
   GD_UP(g.nswp + Rp, zero.data(), Rp * sizeof(int));
   GD_UP(g.rec_n, zero.data(), Rp * sizeof(int));
+  GD_UP(g.rec_n + Rp, zero.data(), Rp * sizeof(int));
   GD_UP(g.mm, zf.data(), (size_t)6 * Rp * sizeof(float));
   GD_UP(g.prop, zf.data(), (size_t)6 * Rp * sizeof(float));
   GD_UP(g.en, zf.data(), (size_t)10 * Rp * sizeof(float));
@@ -822,7 +826,12 @@ gd_status gd_device_info(char *name, int name_len, int *n_cu_out, int *clock_khz
   hipDeviceProp_t prop;
   hipError_t e = hipGetDeviceProperties(&prop, g_device);
   if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
-  if (name && name_len > 0) std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  if (name && name_len > 0) {
+    if (prop.name[0])
+      std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    else  // the runtime reports no marketing name: the architecture alone
+      std::snprintf(name, name_len, "%s, %d CUs", prop.gcnArchName, prop.multiProcessorCount);
+  }
   if (n_cu_out) *n_cu_out = prop.multiProcessorCount;
   if (clock_khz) *clock_khz = prop.clockRate;
   if (hbm) *hbm = prop.totalGlobalMem;
@@ -885,6 +894,8 @@ gd_status gd_job_destroy(gd_job *j) {
   if (j->d_job) hipFree(j->d_job);
   if (j->d_lists) hipFree(j->d_lists);
   if (j->d_rec) hipFree(j->d_rec);
+  if (j->h_rec) hipHostFree(j->h_rec);
+  if (j->copy_stream) hipStreamDestroy(j->copy_stream);
   if (j->stream) hipStreamDestroy(j->stream);
   delete j;
   return GD_OK;
@@ -1007,17 +1018,36 @@ gd_status gd_energy_eval(gd_job *j, int n, const int *conf, const float *pose, f
 // initial-energy pass, then steps_total steps in dump segments; temperature
 // exchange every steps_per_exchange steps; with mc->record the accept records
 // of the last segment stay on the device for gd_dock_records.
-gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stats *stats) {
+// Receiver of the per-dump accept records (gd_dock_stream); null: the records
+// stay on the device (gd_dock_records reads the last dump).
+struct RecSink {
+  gd_record_fn fn;
+  void *user;
+  bool blocking;
+};
+
+// The plain driver: initial-energy pass, then steps_total MC steps in dumps of
+// steps_per_dump (Dock, backend_gpu.cu:64-133), exchange points every
+// steps_per_exchange steps.  With a sink, dump k's accept records (ring half
+// k mod 2) go to the host after the dump's kernels: blocking, as the
+// reference's cudaMemcpy after each dump (backend_gpu.cu:110-116), or on the
+// copy stream while the device runs dump k + 1, handed to the sink once
+// copied; dump k + 2 reuses the half only after the host has taken it.
+static gd_status dock_impl(gd_job *j, const gd_mc_param *mc, const RecSink *sink,
+                           gd_replica_out *out, gd_stats *stats) {
   if (!j || !mc || mc->steps_total < 0 || mc->steps_per_exchange < 0) return GD_ERR_ARG;
   gd_status st = ensure_device();
   if (st != GD_OK) return st;
   GdJob &g = j->h;
   hipStream_t s = j->stream;
   set_mc(g, mc);
+  const bool record = mc->record || sink;
+  g.record = record ? 1 : 0;
   const int spd = mc->steps_per_dump > 0 ? mc->steps_per_dump : std::max(1, mc->steps_total);
-  // record ring: one segment of steps_per_dump entries per replica
-  if (mc->record) {
-    const size_t need = (size_t)g.Rp * spd * GD_REC_W * sizeof(float);
+  const size_t half_floats = (size_t)g.Rp * spd * GD_REC_W;
+  j->last_half = -1;
+  if (record) {
+    const size_t need = 2 * half_floats * sizeof(float);
     if (need > j->rec_bytes) {
       if (j->d_rec) hipFree(j->d_rec);
       j->d_rec = nullptr;
@@ -1032,6 +1062,49 @@ gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stat
     g.rec = nullptr;
     g.rec_cap = 0;
   }
+  // host side of a streamed run: pinned halves (records, then counts) and the copy stream
+  float *h_half[2] = {nullptr, nullptr};
+  int *h_cnt[2] = {nullptr, nullptr};
+  hipEvent_t ev_done[2] = {nullptr, nullptr}, ev_copied[2] = {nullptr, nullptr};
+  if (sink) {
+    const size_t hb = 2 * (half_floats * sizeof(float) + (size_t)g.Rp * sizeof(int));
+    if (hb > j->h_rec_bytes) {
+      if (j->h_rec) hipHostFree(j->h_rec);
+      j->h_rec = nullptr;
+      j->h_rec_bytes = 0;
+      hipError_t e = hipHostMalloc(&j->h_rec, hb, hipHostMallocDefault);
+      if (e != hipSuccess) return hip_fail(e, "hipHostMalloc records");
+      j->h_rec_bytes = hb;
+    }
+    for (int b = 0; b < 2; ++b) {
+      h_half[b] = (float *)j->h_rec + b * half_floats;
+      h_cnt[b] = (int *)((float *)j->h_rec + 2 * half_floats) + (size_t)b * g.Rp;
+    }
+    if (!sink->blocking) {
+      hipError_t e = hipSuccess;
+      if (!j->copy_stream) e = hipStreamCreateWithFlags(&j->copy_stream, hipStreamNonBlocking);
+      for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+        e = hipEventCreateWithFlags(&ev_done[b], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_copied[b], hipEventDisableTiming);
+      }
+      if (e != hipSuccess) {
+        for (int b = 0; b < 2; ++b) {
+          if (ev_done[b]) hipEventDestroy(ev_done[b]);
+          if (ev_copied[b]) hipEventDestroy(ev_copied[b]);
+        }
+        return hip_fail(e, "record stream set-up");
+      }
+    }
+  }
+  struct EvGuard {
+    hipEvent_t *a, *b;
+    ~EvGuard() {
+      for (int k = 0; k < 2; ++k) {
+        if (a[k]) hipEventDestroy(a[k]);
+        if (b[k]) hipEventDestroy(b[k]);
+      }
+    }
+  } ev_guard{ev_done, ev_copied};
   if ((st = reset_replicas(j)) != GD_OK) return st;
   hipError_t e = hipMemcpyAsync(j->d_job, &g, sizeof(GdJob), hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync job");
@@ -1066,6 +1139,10 @@ gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stat
   const std::function<gd_status()> pair = [&]() {
     return launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s);
   };
+  // hand dump k (ring half b, copied to the host) to the sink
+  const auto deliver = [&](int k, int b) {
+    sink->fn(sink->user, k, k * spd, g.R, spd, h_half[b], h_cnt[b]);
+  };
   KTimer tp;
   const bool timing = stats != nullptr;
   const auto t0 = std::chrono::steady_clock::now();
@@ -1073,9 +1150,11 @@ gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stat
   if (st == GD_OK) st = launch_transform(j->d_job, j->d_blocks, j->n_blocks, s);
   if (st == GD_OK) st = launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s);
   if (st == GD_OK) st = launch_finish(j->d_job, j->d_blocks, j->n_blocks, GD_MODE_INIT, 0, s);
-  for (int s1 = 0; s1 < mc->steps_total && st == GD_OK; s1 += spd) {
-    if (mc->record) {
-      e = hipMemsetAsync(g.rec_n, 0, (size_t)g.Rp * sizeof(int), s);
+  int pending = -1;  // streamed dump whose copy is in flight
+  for (int s1 = 0, k = 0; s1 < mc->steps_total && st == GD_OK; s1 += spd, ++k) {
+    const int b = k & 1;
+    if (record) {
+      e = hipMemsetAsync(g.rec_n + (size_t)b * g.Rp, 0, (size_t)g.Rp * sizeof(int), s);
       if (e != hipSuccess) {
         st = hip_fail(e, "hipMemsetAsync");
         break;
@@ -1089,24 +1168,61 @@ gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stat
       if (timed && (st = tp.end(s)) != GD_OK) break;
       if ((st = launch_finish(j->d_job, j->d_blocks, j->n_blocks, GD_MODE_STEP, step, s)) != GD_OK) break;
       if (n_rounds > 0 && (step + 1) % spe == 0 && (step + 1) / spe <= n_rounds) {
-        const int k = (step + 1) / spe - 1;
-        if (tx && (st = launch_exchange(j->d_job, d_ex + k, 1, s)) != GD_OK) break;
+        const int r = (step + 1) / spe - 1;
+        if (tx && (st = launch_exchange(j->d_job, d_ex + r, 1, s)) != GD_OK) break;
         for (int axis = 0; axis < 2; ++axis)
           if ((axis ? px : lx) &&
-              (st = launch_swap(axis, j->d_job, j->d_blocks, j->n_blocks, d_swr + k, step, nullptr,
+              (st = launch_swap(axis, j->d_job, j->d_blocks, j->n_blocks, d_swr + r, step, nullptr,
                                 nullptr, pair, s)) != GD_OK)
             break;
         if (st != GD_OK) break;
       }
     }
-    e = hipStreamSynchronize(s);  // segment boundary (the reference's blocking D2H)
-    if (st == GD_OK && e != hipSuccess) st = hip_fail(e, "MC segment");
-    if (st == GD_OK && timing) st = tp.collect();
+    if (st != GD_OK) break;
+    j->last_half = record ? b : -1;
+    if (!sink) continue;
+    const float *d_half = g.rec + (size_t)b * half_floats;
+    const int *d_cnt = g.rec_n + (size_t)b * g.Rp;
+    if (sink->blocking) {  // the reference's shape: wait, copy, hand over
+      if ((e = hipStreamSynchronize(s)) != hipSuccess ||
+          (e = hipMemcpy(h_half[b], d_half, half_floats * sizeof(float), hipMemcpyDeviceToHost)) !=
+              hipSuccess ||
+          (e = hipMemcpy(h_cnt[b], d_cnt, (size_t)g.Rp * sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) {
+        st = hip_fail(e, "record copy");
+        break;
+      }
+      deliver(k, b);
+      continue;
+    }
+    if ((e = hipEventRecord(ev_done[b], s)) != hipSuccess ||
+        (e = hipStreamWaitEvent(j->copy_stream, ev_done[b], 0)) != hipSuccess ||
+        (e = hipMemcpyAsync(h_half[b], d_half, half_floats * sizeof(float), hipMemcpyDeviceToHost,
+                            j->copy_stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h_cnt[b], d_cnt, (size_t)g.Rp * sizeof(int), hipMemcpyDeviceToHost,
+                            j->copy_stream)) != hipSuccess ||
+        (e = hipEventRecord(ev_copied[b], j->copy_stream)) != hipSuccess) {
+      st = hip_fail(e, "record stream");
+      break;
+    }
+    if (pending >= 0) {  // the previous dump, copied while this one ran
+      if ((e = hipEventSynchronize(ev_copied[pending & 1])) != hipSuccess) {
+        st = hip_fail(e, "record copy");
+        break;
+      }
+      deliver(pending, pending & 1);
+    }
+    pending = k;
   }
   {
     hipError_t e2 = hipStreamSynchronize(s);  // also covers steps_total == 0
     if (st == GD_OK && e2 != hipSuccess) st = hip_fail(e2, "hipStreamSynchronize");
+    if (j->copy_stream) {
+      e2 = hipStreamSynchronize(j->copy_stream);
+      if (st == GD_OK && e2 != hipSuccess) st = hip_fail(e2, "record copy");
+    }
   }
+  if (st == GD_OK && sink && !sink->blocking && pending >= 0) deliver(pending, pending & 1);
+  if (st == GD_OK && timing) st = tp.collect();
   const auto t1 = std::chrono::steady_clock::now();
   if (d_xt) hipFree(d_xt);
   if (st != GD_OK) return st;
@@ -1121,6 +1237,17 @@ gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stat
   return read_out(j, out);
 }
 
+gd_status gd_dock(gd_job *j, const gd_mc_param *mc, gd_replica_out *out, gd_stats *stats) {
+  return dock_impl(j, mc, nullptr, out, stats);
+}
+
+gd_status gd_dock_stream(gd_job *j, const gd_mc_param *mc, gd_record_fn fn, void *user, int flags,
+                         gd_replica_out *out, gd_stats *stats) {
+  if (!fn || (flags & ~GD_STREAM_BLOCKING)) return GD_ERR_ARG;
+  const RecSink sink = {fn, user, (flags & GD_STREAM_BLOCKING) != 0};
+  return dock_impl(j, mc, &sink, out, stats);
+}
+
 gd_status gd_job_replicas(gd_job *j, gd_replica_out *out) {
   if (!j || !out) return GD_ERR_ARG;
   return gdr::read_out(j, out);
@@ -1129,15 +1256,18 @@ gd_status gd_job_replicas(gd_job *j, gd_replica_out *out) {
 gd_status gd_dock_records(gd_job *j, int cap, float *rec, int *count) {
   if (!j || cap < 0 || !count || (cap > 0 && !rec)) return GD_ERR_ARG;
   const GdJob &g = j->h;
-  if (!g.rec) return GD_ERR_STATE;
+  if (!g.rec || j->last_half < 0) return GD_ERR_STATE;
   const int R = g.R, Rp = g.Rp, rc = g.rec_cap;
+  const size_t half = (size_t)j->last_half;
   std::vector<int> n(Rp);
   hipError_t e = hipStreamSynchronize(j->stream);
-  if (e == hipSuccess) e = hipMemcpy(n.data(), g.rec_n, (size_t)Rp * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    e = hipMemcpy(n.data(), g.rec_n + half * Rp, (size_t)Rp * sizeof(int), hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpy rec_n");
   std::vector<float> buf((size_t)Rp * rc * GD_REC_W);
   if (!buf.empty()) {
-    e = hipMemcpy(buf.data(), g.rec, buf.size() * sizeof(float), hipMemcpyDeviceToHost);
+    e = hipMemcpy(buf.data(), g.rec + half * buf.size(), buf.size() * sizeof(float),
+                  hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpy records");
   }
   for (int r = 0; r < R; ++r) {
@@ -1146,6 +1276,136 @@ gd_status gd_dock_records(gd_job *j, int cap, float *rec, int *count) {
     for (int k = 0; k < m; ++k)
       for (int w = 0; w < GD_REC_W; ++w)
         rec[((size_t)r * cap + k) * GD_REC_W + w] = buf[((size_t)r * rc + k) * GD_REC_W + w];
+  }
+  return GD_OK;
+}
+
+// Move increments as gd_dock draws them (statistical tests of the move
+// generator): inc[s][r][6] for replicas r < n_rep at steps step0 + s,
+// s < n_steps, computed on the device (on_device != 0, the kernels' own code)
+// or on the host (the same source compiled for the host).
+gd_status gd_move_draws(uint64_t seed, uint32_t job_id, int move_mode, const float move_scale[6],
+                        int n_rep, uint32_t step0, int n_steps, int on_device, float *inc) {
+  if (!move_scale || !inc || n_rep <= 0 || n_steps <= 0 ||
+      (move_mode != GD_MOVE_REF_UNIFORM && move_mode != GD_MOVE_GAUSSIAN))
+    return GD_ERR_ARG;
+  const long long n = (long long)n_rep * n_steps;
+  if (n > (1ll << 28)) return GD_ERR_SIZE;
+  if (!on_device) {
+    for (long long i = 0; i < n; ++i) {
+      uint32_t w[8];
+      gd_step_draws(seed, job_id, (uint32_t)(i % n_rep), step0 + (uint32_t)(i / n_rep), w);
+      gd_move_incr(move_mode, move_scale, w, inc + i * 6);
+    }
+    return GD_OK;
+  }
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  GdScale6 sc;
+  for (int k = 0; k < 6; ++k) sc.v[k] = move_scale[k];
+  float *d = nullptr;
+  hipError_t e = hipMalloc(&d, (size_t)n * 6 * sizeof(float));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc draws");
+  hipLaunchKernelGGL(gd_move_draws_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, seed,
+                     job_id, move_mode, sc, n_rep, step0, (int)n, d);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(inc, d, (size_t)n * 6 * sizeof(float), hipMemcpyDeviceToHost);
+  hipFree(d);
+  return e == hipSuccess ? GD_OK : hip_fail(e, "gd_move_draws_kernel");
+}
+
+// Increments from given draw words: w[n][8] -> inc[n][6] (host code path).
+gd_status gd_move_incr_words(int move_mode, const float move_scale[6], int n, const uint32_t *w,
+                             float *inc) {
+  if (!move_scale || !w || !inc || n < 0 ||
+      (move_mode != GD_MOVE_REF_UNIFORM && move_mode != GD_MOVE_GAUSSIAN))
+    return GD_ERR_ARG;
+  for (int i = 0; i < n; ++i) gd_move_incr(move_mode, move_scale, w + (size_t)i * 8, inc + (size_t)i * 6);
+  return GD_OK;
+}
+
+// Engine clock under load: every SIMD of every CU runs dependent FMAs for
+// about `ms` milliseconds; the shader-clock counter's rate against the
+// constant-rate counter (hipDeviceAttributeWallClockRate) is the clock.
+gd_status gd_measure_sclk(int ms, double *sclk_mhz) {
+  if (!sclk_mhz || ms <= 0 || ms > 2000) return GD_ERR_ARG;
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  int wall_khz = 0;
+  hipError_t e = hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, g_device);
+  if (e != hipSuccess || wall_khz <= 0) return hip_fail(e, "hipDeviceAttributeWallClockRate");
+  long long *d = nullptr;
+  e = hipMalloc(&d, 2 * sizeof(long long) + sizeof(float));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc clock");
+  const long long ticks = (long long)wall_khz * ms;
+  hipLaunchKernelGGL(gd_clock_kernel, dim3(4 * (g_cu > 0 ? g_cu : 256)), dim3(64), 0, 0, ticks, 1.0f, d,
+                     (float *)(d + 2));
+  long long h[2] = {0, 0};
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "gd_clock_kernel");
+  if (h[1] <= 0) return GD_ERR_STATE;
+  *sclk_mhz = (double)h[0] / (double)h[1] * (double)wall_khz / 1000.0;
+  return GD_OK;
+}
+
+// PrintRecord-format text sink of gd_dock_stream (util_print.C:70-151).
+struct gd_record_csv {
+  FILE *f = nullptr;
+  int lo = 0, hi = -1;
+  bool failed = false;
+};
+
+gd_status gd_record_csv_open(const char *path, int rep_lo, int rep_hi, int header,
+                             gd_record_csv **out) {
+  if (!path || !out || rep_lo < 0 || rep_hi < rep_lo) return GD_ERR_ARG;
+  *out = nullptr;
+  FILE *f = std::fopen(path, "w");
+  if (!f) {
+    set_error(std::string("cannot open ") + path);
+    return GD_ERR_IO;
+  }
+  gd_record_csv *w = new gd_record_csv();
+  w->f = f;
+  w->lo = rep_lo;
+  w->hi = rep_hi;
+  if (header) {
+    static const char *names[10] = {"vdw", "ele", "pmf", "psp", "hdb", "hpc", "kde", "lhm", "dst", "total"};
+    std::fprintf(f, "rep step");
+    for (int i = 0; i < 10; ++i) std::fprintf(f, " %s", names[i]);
+    std::fprintf(f, "\n");
+  }
+  *out = w;
+  return GD_OK;
+}
+
+void gd_record_csv_write(void *writer, int dump, int step_begin, int n_rep, int cap,
+                         const float *rec, const int *count) {
+  (void)dump;
+  (void)step_begin;
+  gd_record_csv *w = (gd_record_csv *)writer;
+  if (!w || !w->f || !rec || !count) return;
+  for (int r = w->lo; r <= w->hi && r < n_rep; ++r) {
+    const int n = std::min(count[r], cap);
+    for (int k = 0; k < n; ++k) {
+      const float *o = rec + ((size_t)r * cap + k) * GD_REC_W;
+      if (std::fprintf(w->f, "%3d %d", r, (int)o[0]) < 0) w->failed = true;
+      for (int i = 0; i < 10; ++i)
+        if (std::fprintf(w->f, " %.4f", o[7 + i]) < 0) w->failed = true;
+      if (std::fprintf(w->f, "\n") < 0) w->failed = true;
+    }
+  }
+}
+
+gd_status gd_record_csv_close(gd_record_csv *w) {
+  if (!w) return GD_ERR_ARG;
+  bool bad = w->failed;
+  if (w->f && std::fclose(w->f) != 0) bad = true;
+  delete w;
+  if (bad) {
+    set_error("record file write failed");
+    return GD_ERR_IO;
   }
   return GD_OK;
 }

@@ -221,6 +221,36 @@ class DeviceJob:
                             C.byref(st) if st is not None else None), "gd_dock")
         return out, st
 
+    def dock_stream(self, mc, on_dump, blocking=False, stats=True):
+        """gd_dock_stream: on_dump(dump, step_begin, rec, count) receives each
+        dump's accept records as numpy copies, rec[n_rep][cap][17] and
+        count[n_rep] (cap = steps_per_dump)."""
+        def fn(_user, dump, step_begin, n_rep, cap, rec, count):
+            r = np.ctypeslib.as_array(rec, (n_rep * cap * abi.REC_W,)).reshape(n_rep, cap, abi.REC_W)
+            c = np.ctypeslib.as_array(count, (n_rep,))
+            on_dump(dump, step_begin, r.copy(), c.copy())
+        cb = abi.RECORD_FN(fn)
+        out = ReplicaOut(self.n_rep)
+        st = abi.gd_stats() if stats else None
+        check(lib().gd_dock_stream(self.h, C.byref(mc), C.cast(cb, C.c_void_p), None,
+                                   abi.GD_STREAM_BLOCKING if blocking else 0, C.byref(out.c),
+                                   C.byref(st) if st is not None else None), "gd_dock_stream")
+        return out, st
+
+    def dock_to_csv(self, mc, path, rep_lo=0, rep_hi=0, header=True, blocking=False):
+        """gd_dock_stream into the library's PrintRecord-format file sink."""
+        w = C.c_void_p()
+        check(lib().gd_record_csv_open(os.fsencode(path), rep_lo, rep_hi, 1 if header else 0,
+                                       C.byref(w)), "gd_record_csv_open")
+        out = ReplicaOut(self.n_rep)
+        fnp = C.cast(lib().gd_record_csv_write, C.c_void_p)
+        st = lib().gd_dock_stream(self.h, C.byref(mc), fnp, w, abi.GD_STREAM_BLOCKING if blocking else 0,
+                                  C.byref(out.c), None)
+        st2 = lib().gd_record_csv_close(w)
+        check(st, "gd_dock_stream")
+        check(st2, "gd_record_csv_close")
+        return out
+
     def replicas(self):
         """Current device replica state (gd_job_replicas)."""
         out = ReplicaOut(self.n_rep)

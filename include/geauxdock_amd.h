@@ -195,8 +195,45 @@ gd_status gd_dock(gd_job *job, const gd_mc_param *mc, gd_replica_out *out, gd_st
 gd_status gd_job_replicas(gd_job *job, gd_replica_out *out);
 
 /* Accept records of the last dump segment: rec[n_rep][cap] entries of
- * (step, movematrix[6], energy[10]) packed as 17 floats; count[n_rep]. */
+ * (step, movematrix[6], energy[10]) packed as 17 floats; count[n_rep].
+ * A record holds a replica's accepted state at the start of step `step`,
+ * kept when the step before it (or the initial pass, or a conformer swap)
+ * was accepted (kernel_cpp_montecarlo_soa.C:224-231); at most
+ * steps_per_dump per replica per dump (the reference's Record,
+ * geauxdock.h:374-378).  GD_ERR_STATE when no dump has recorded. */
 gd_status gd_dock_records(gd_job *job, int cap, float *rec, int *count);
+
+/* Receiver of one dump's records (gd_dock_stream): dump `dump` covers MC
+ * steps [step_begin, step_begin + cap); rec points at n_rep rows of
+ * cap * 17 floats (the gd_dock_records layout with cap = steps_per_dump),
+ * count[n_rep] gives the entries used in each row.  Called on the thread that
+ * runs gd_dock_stream, once per dump and in dump order; the arrays are valid
+ * only during the call, and the callback must not call the library on the
+ * same job. */
+typedef void (*gd_record_fn)(void *user, int dump, int step_begin, int n_rep, int cap,
+                             const float *rec, const int *count);
+enum { GD_STREAM_BLOCKING = 1 };
+/* gd_dock with every dump's accept records handed to fn (records on
+ * whatever mc->record says).  By default each dump's records are copied to
+ * pinned host memory on a second stream while the device runs the next dump;
+ * GD_STREAM_BLOCKING waits for the dump and copies synchronously instead, the
+ * reference's per-dump cudaMemcpy (backend_gpu.cu:96-118) -- the plain path in
+ * the reference's shape. */
+gd_status gd_dock_stream(gd_job *job, const gd_mc_param *mc, gd_record_fn fn, void *user, int flags,
+                         gd_replica_out *out, gd_stats *stats);
+
+/* A ready-made record sink: appends PrintRecord's lines (util_print.C:70-151:
+ * "%3d %d" replica and step, then " %.4f" per energy term) for replicas
+ * rep_lo..rep_hi of every dump to a text file; header != 0 writes the
+ * "rep step vdw ... total" title line first.  Pass gd_record_csv_write as
+ * the gd_record_fn and the writer as its user pointer. */
+typedef struct gd_record_csv gd_record_csv;
+gd_status gd_record_csv_open(const char *path, int rep_lo, int rep_hi, int header,
+                             gd_record_csv **out);
+void gd_record_csv_write(void *writer, int dump, int step_begin, int n_rep, int cap,
+                         const float *rec, const int *count);
+/* Closes the file; returns GD_ERR_IO if any write failed. */
+gd_status gd_record_csv_close(gd_record_csv *writer);
 
 /* Many complexes in one launch sequence (persistent work-queue pair kernel,
  * graph-captured sweep segments).  mc[i] / out[i] per job. */
@@ -298,6 +335,26 @@ gd_status gd_host_job_desc(const gd_host_job *h, gd_job_desc *desc);
 gd_status gd_host_job_free(gd_host_job *h);
 /* Conformer RMSD after optimal superposition (replaces rmsd.f u3b, load.C:421). */
 float gd_kabsch_rmsd(int n, const float *a_xyz, const float *b_xyz);
+
+/* ---- diagnostics (tests and measurement; no reference counterpart) ------ */
+/* Philox4x32-R of one counter block (known-answer tests). */
+void gd_philox(const uint32_t ctr[4], const uint32_t key[2], int rounds, uint32_t out[4]);
+/* Move increments as gd_dock draws them: inc[s][r][6] for replicas r < n_rep
+ * at MC steps step0 + s, s < n_steps (at most 2^28 replica-steps), computed by
+ * the device kernels' code (on_device != 0) or by the same source compiled for
+ * the host.  The reference move it generalises: kernel_cpp_montecarlo_soa.C:242-250. */
+gd_status gd_move_draws(uint64_t seed, uint32_t job_id, int move_mode, const float move_scale[6],
+                        int n_rep, uint32_t step0, int n_steps, int on_device, float *inc);
+/* Increments from given Philox words: w[n][8] -> inc[n][6] (host code path). */
+gd_status gd_move_incr_words(int move_mode, const float move_scale[6], int n, const uint32_t *w,
+                             float *inc);
+/* Engine clock under an all-CU FMA load, in MHz, measured over about `ms`
+ * milliseconds (1..2000) from the shader-clock and constant-rate counters. */
+gd_status gd_measure_sclk(int ms, double *sclk_mhz);
+/* The pair kernel's far-field and near-branch r^2 thresholds of a
+ * (ligand type, point type) parameter row (tests of the exact-zero shortcut). */
+float gd_far_r2(float pmf0, float el1);
+float gd_near_r2(float pmf0, float el1, float hdb0, float hdb1);
 
 #ifdef __cplusplus
 }

@@ -24,8 +24,9 @@
  *   void Dock(Complex*, Record*, Complex**, Record**, ParaT**, gd_rng_state**);
  *                                                         backend_gpu.h:18-23
  * gd_rng_state (a Philox key + counter base) takes the place of cuRAND's
- * curandState; the typedef below lets main_gpu keep its declarations.  As in
- * the reference, a failing HIP call prints the error and exits.
+ * curandState: a caller declares its RNG pointers as gd_rng_state* (no CUDA
+ * type name is defined here).  As in the reference, a failing HIP call prints
+ * the error and exits.
  */
 #ifndef GEAUXDOCK_DROPIN_H
 #define GEAUXDOCK_DROPIN_H
@@ -211,7 +212,6 @@ struct gd_rng_state {
   uint32_t job_id;
   uint32_t reserved;
 };
-typedef struct gd_rng_state curandState;
 
 void SetDevice();
 void DeviceAlloc(Complex **c, Record **r, gd_rng_state **s);

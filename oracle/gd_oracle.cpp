@@ -245,8 +245,21 @@ void energy(const Job &J, int prt, const float m[6], const T *xf, const T *yf, c
       const T hdb0 = e->hdb0[lt][pt];
       if (hdb0 > (T)0.1f) {  // hydrogen bond, :116-121
         const T hdb1 = e->hdb1[lt][pt];
-        const T h3 = (dst - hdb0) * hdb1;
-        const T v = hdb1 * t_exp((T)-0.5f * h3 * h3);
+        T v;
+        if (literal || sizeof(T) == 8) {
+          const T h3 = (dst - hdb0) * hdb1;
+          v = hdb1 * t_exp((T)-0.5f * h3 * h3);
+        } else {
+          // GPU rounding: the exponent q = log2(e)/2 ((r - hdb0) hdb1)^2 in FP64
+          // from exact coordinate differences, 2^-q = 2^-floor(q) 2^-frac(q)
+          const double ex = (double)lx - (double)px[p], ey = (double)ly - (double)py[p],
+                       ez = (double)lz - (double)pz[p];
+          const double rd = sqrt(ex * ex + ey * ey + ez * ez);
+          const double hd = (rd - (double)e->hdb0[lt][pt]) * (double)e->hdb1[lt][pt];
+          const double q = hd * hd * 0.72134752044448170368;
+          const double fl = floor(q);
+          v = hdb1 * ldexpf(exp2f(-(float)(q - fl)), -(int)fl);
+        }
         ehdb += v;
         sc[4] += fabs((double)v);
       }
@@ -440,6 +453,20 @@ void or_philox(const uint32_t ctr[4], const uint32_t key[2], int rounds, uint32_
   for (int i = 0; i < 4; ++i) out[i] = r.v[i];
 }
 
+// Move increments of the oracle's chains: inc[s][r][6] for replicas r < n_rep
+// at steps step0 + s (the statistical tests of the move generator).
+void or_move_draws(uint64_t seed, uint32_t job, int move_mode, const float scale[6], int n_rep,
+                   uint32_t step0, int n_steps, float *inc) {
+  gd_mc_param mc;
+  memset(&mc, 0, sizeof mc);
+  mc.move_mode = move_mode;
+  for (int k = 0; k < 6; ++k) mc.move_scale[k] = scale[k];
+  for (long long i = 0; i < (long long)n_rep * n_steps; ++i) {
+    const Draws d = draws(seed, job, (uint32_t)(i % n_rep), step0 + (uint32_t)(i / n_rep));
+    move_incr(&mc, d, inc + i * 6);
+  }
+}
+
 void or_step_draws(uint64_t seed, uint32_t job, uint32_t rep, uint32_t step, uint32_t w[8]) {
   const Draws d = draws(seed, job, rep, step);
   for (int i = 0; i < 8; ++i) w[i] = d.w[i];
@@ -519,8 +546,14 @@ int or_energy(const gd_job_desc *d, int n, const int *conf, const float *pose,
 // rep_lo/rep_hi restrict the run to a replica range (for bounded timing
 // samples; exchange then needs the whole range of each ladder).  Optional
 // trace[n_rep][steps_total] receives the accepted total energy after each step.
+// Optional rec[n_rep][rec_cap][17] / rec_count[n_rep] receive the accept
+// records of the whole run (kernel_cpp_montecarlo_soa.C:224-231, never reset):
+// at the start of step s, the replica's accepted (step s, movematrix,
+// energy) when the step before it, the initial pass or a conformer swap was
+// accepted.
 int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_lo, int rep_hi,
-            gd_replica_out *out, float *trace, double *ms) {
+            gd_replica_out *out, float *trace, double *ms, float *rec, int rec_cap,
+            int *rec_count) {
   if (!d || !mc) return GD_ERR_ARG;
   const Job J = make_job(d);
   const int L = J.L;
@@ -540,6 +573,9 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
         owner[((size_t)p * nl + l) * nt + t] = r;
       }
   if (nthreads > 0) omp_set_num_threads(nthreads);
+  if (rec_count)
+    for (int r = 0; r < R; ++r) rec_count[r] = 0;
+  if (!rec_count) rec = nullptr;
   const auto t0 = std::chrono::steady_clock::now();
   const int seg = mc->steps_per_exchange > 0 ? mc->steps_per_exchange : std::max(1, mc->steps_total);
   // initial energy, forced accept (backend_cpu_mic.C:55)
@@ -554,6 +590,13 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
       const int nsteps = init ? 1 : s_end - s_begin;
       for (int k = 0; k < nsteps; ++k) {
         const int s = s_begin + k;
+        if (!init && rec && acc[r] && rec_count[r] < rec_cap) {
+          float *o = rec + ((size_t)r * rec_cap + rec_count[r]) * 17;
+          o[0] = (float)s;
+          for (int i = 0; i < 6; ++i) o[1 + i] = mm[(size_t)r * 6 + i];
+          for (int i = 0; i < 10; ++i) o[7 + i] = en[(size_t)r * 10 + i];
+          ++rec_count[r];
+        }
         float m[6], u = 0.0f;
         if (init) {
           for (int i = 0; i < 6; ++i) m[i] = mm[(size_t)r * 6 + i];
@@ -659,7 +702,8 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
 // range, every host thread, the Philox stream of mc->seed / mc->job_id.
 int gd_cpu_dock(const gd_job_desc *d, const gd_mc_param *mc, gd_replica_out *out) {
   if (!d) return GD_ERR_ARG;
-  return or_dock(d, mc, 0, 0, d->n_lig * d->n_prt * d->n_tmp, out, nullptr, nullptr);
+  return or_dock(d, mc, 0, 0, d->n_lig * d->n_prt * d->n_tmp, out, nullptr, nullptr, nullptr, 0,
+                 nullptr);
 }
 
 // Smallest float x with expf(x) > 0: the forced initial accept of the

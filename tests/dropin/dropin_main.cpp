@@ -140,13 +140,17 @@ int main(int argc, char **argv) {
   fill(d, ene, ch);
   ch->mcpara.steps_total = steps;
   ch->mcpara.steps_per_dump = steps < GD_REF_STEPS_PER_DUMP ? (steps > 0 ? steps : 1) : GD_REF_STEPS_PER_DUMP;
-  ch->mcpara.steps_per_exchange = spe;
+  // the reference frontend never writes this field (McPara is allocated
+  // uninitialised): leave a leftover value in it; the exchange period is given
+  // through GD_EXCHANGE_STEPS instead
+  ch->mcpara.steps_per_exchange = 3;
+  if (spe > 0) setenv("GD_EXCHANGE_STEPS", std::to_string(spe).c_str(), 1);
   const float scale[6] = {0.02f, 0.02f, 0.02f, 0.08f, 0.08f, 0.08f};  // the reference defaults
   for (int k = 0; k < 6; ++k) ch->mcpara.move_scale[k] = scale[k];
 
   Complex *cd[1];
   Record *rd[1];
-  curandState *sd[1];
+  gd_rng_state *sd[1];
   ParaT parat[1];
   ParaT *pt[1] = {&parat[0]};
   SetDevice();

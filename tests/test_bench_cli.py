@@ -26,3 +26,34 @@ def test_algorithmic_flops_of_the_benchmark_job():
     assert pairs == job.n_rep * bench.L * bench.P
     assert 62 * pairs < flops < 1.1 * 62 * pairs
     assert abs(flops - 1.386e10) / 1.386e10 < 0.01
+
+
+def test_rates_of_a_timed_run():
+    """2,000 MC steps of the 1,760-replica job in 0.259 s: 1.359e7
+    replica-steps/s and 0.2 jobs in 0.259 s = 0.772 jobs/s (one job is the
+    whole 10,000-step run)."""
+    sys.path.insert(0, ROOT)
+    import bench
+    rs, jobs = bench.c2_rates(1760, 2000, 0.259)
+    assert abs(rs - 1760 * 2000 / 0.259) < 1e-6 * rs
+    assert abs(jobs - 0.2 / 0.259) < 1e-12
+    rs, jobs = bench.c2_rates(1760, 10000, 1.3)   # the whole job
+    assert abs(jobs - 1 / 1.3) < 1e-12
+
+
+def test_workload_names_the_steps_timed():
+    sys.path.insert(0, ROOT)
+    import bench
+    w = bench.c2_workload(2000)
+    assert "2000 MC steps of the 10000-step job timed" in w
+    assert "1760 replicas" in w and "P=3000" in w and "L=40" in w
+
+
+def test_cpu_leg_describes_its_cores():
+    sys.path.insert(0, ROOT)
+    import bench
+    logical, phys, phys_all = bench.host_cores()
+    assert 1 <= phys <= logical and phys <= phys_all
+    f = bench.cpu_fields(bench.cpu_threads())
+    assert 1 <= f["cores"] <= min(16, phys)
+    assert f["cpu_model"] and f["cpu_model"] != "unknown"

@@ -150,6 +150,99 @@ def test_records(small_job):
         assert np.all(np.diff(steps) > 0) and np.all((steps >= 40) & (steps < 60))
 
 
+def _stream(dev, mc, blocking):
+    dumps = []
+    out, _ = dev.dock_stream(mc, lambda k, s0, rec, cnt: dumps.append((k, s0, rec, cnt)),
+                             blocking=blocking)
+    return out, dumps
+
+
+def _flatten(dumps, n_rep):
+    """Per replica, the records of every dump in order: list of [n][17]."""
+    per = [[] for _ in range(n_rep)]
+    for _, _, rec, cnt in dumps:
+        for r in range(n_rep):
+            per[r].extend(rec[r, :cnt[r]])
+    return [np.array(x, np.float32).reshape(-1, 17) for x in per]
+
+
+def test_record_stream_against_oracle(small_job, oracle):
+    """Every dump's records streamed to the host: 90 steps in dumps of 20 (the
+    last one 10 steps) with pose swaps and temperature exchange every 15.
+    Streamed and blocking deliveries give the same bits; the last dump equals
+    gd_dock_records; each record is a true (pose, energy) pair (a fresh
+    evaluation gives the same bits); against the oracle's records of the same
+    draws, the replicas whose chains agree have the same record steps, poses
+    within 1e-5 (the move draws use each side's own libm, a few ulps per
+    accepted move) and terms
+    within 5e-5 -- the oracle's FP32 chain sums each term serially over every
+    pair, and that sum alone is ~2e-5 off FP64 on the PMF term (the records'
+    own energies are held to the FP64 bar through the fresh evaluation)."""
+    dev = g.DeviceJob(small_job)
+    mc = g.make_mc(90, steps_per_dump=20, steps_per_exchange=15, move_mode="gaussian", seed=3,
+                   axes=("temp", "lig", "prt"))
+    out_a, da = _stream(dev, mc, blocking=False)
+    out_b, db = _stream(dev, mc, blocking=True)
+    assert [d[:2] for d in da] == [(k, 20 * k) for k in range(5)]
+    assert [d[:2] for d in db] == [d[:2] for d in da]
+    for x, y in zip(da, db):
+        assert np.array_equal(x[3], y[3])
+        for r in range(small_job.n_rep):
+            assert np.array_equal(x[2][r, :x[3][r]], y[2][r, :y[3][r]])
+    assert np.array_equal(out_a.energy, out_b.energy)
+    rec, cnt = dev.records(20)
+    k, _, rl, cl = da[-1]
+    assert np.array_equal(cnt, cl)
+    for r in range(small_job.n_rep):
+        assert np.array_equal(rec[r, :cnt[r]], rl[r, :cl[r]])
+        assert np.all(cl[r] <= 10)
+    gpu = _flatten(da, small_job.n_rep)
+    # true (pose, energy) pairs, bit for bit
+    d = small_job.desc
+    conf, pose, want = [], [], []
+    for r in range(small_job.n_rep):
+        for row in gpu[r]:
+            conf.append((r % d.n_lig, r // (d.n_tmp * d.n_lig)))
+            pose.append(row[1:7])
+            want.append(row[7:17])
+    assert len(pose) > 0
+    terms, _, _ = dev.eval(np.array(conf, np.int32), np.array(pose, np.float32))
+    assert np.array_equal(terms, np.array(want, np.float32))
+    # the oracle's records over the whole run
+    o_out, _, _, (o_rec, o_cnt) = oracle.dock(small_job, mc, threads=8, records=True)
+    same = 0
+    for r in range(small_job.n_rep):
+        ours = gpu[r]
+        theirs = o_rec[r, :o_cnt[r]]
+        steps = np.all(np.diff(ours[:, 0]) > 0) if len(ours) > 1 else True
+        assert steps and np.all((ours[:, 0] >= 0) & (ours[:, 0] < 90))
+        if len(ours) == len(theirs) and np.array_equal(ours[:, 0], theirs[:, 0]):
+            dm = np.abs(ours[:, 1:7] - theirs[:, 1:7])
+            assert dm.max() <= 1e-5, (r, dm.max())
+            rel = np.abs(ours[:, 7:] - theirs[:, 7:]) / np.maximum(np.abs(theirs[:, 7:]), 1.0)
+            assert rel.max() <= 5e-5, (r, rel.max())
+            same += 1
+    assert same >= 0.9 * small_job.n_rep, same
+
+
+def test_record_csv_sink(small_job, tmp_path):
+    """The PrintRecord-format file sink (util_print.C:70-151): one line per
+    record of replicas 0..3, '%3d %d' then ' %.4f' per term."""
+    dev = g.DeviceJob(small_job)
+    mc = g.make_mc(40, steps_per_dump=20, move_mode="gaussian", seed=5)
+    out, dumps = _stream(dev, mc, blocking=False)
+    path = tmp_path / "rec.txt"
+    dev.dock_to_csv(mc, str(path), rep_lo=0, rep_hi=3, header=True)
+    lines = path.read_text().splitlines()
+    assert lines[0] == "rep step vdw ele pmf psp hdb hpc kde lhm dst total"
+    want = []
+    for _, _, rec, cnt in dumps:
+        for r in range(4):
+            for row in rec[r, :cnt[r]]:
+                want.append("%3d %d" % (r, int(row[0])) + "".join(" %.4f" % x for x in row[7:17]))
+    assert lines[1:] == want
+
+
 def test_dock_is_deterministic(small_job):
     dev = g.DeviceJob(small_job)
     mc = g.make_mc(20, move_mode="gaussian", seed=21)

@@ -66,3 +66,30 @@ def test_dropin_dock_with_exchange():
             assert sorted(tmp[reps]) == list(range(nt))
             moved += sum(tmp[r] != t for t, r in enumerate(reps))
     assert moved > 0
+
+
+def test_dropin_exchange_off_unless_requested():
+    """The frontend never writes McPara.steps_per_exchange; the test driver
+    leaves a leftover value (3) in it.  Without GD_EXCHANGE_STEPS the drop-in
+    Dock must not exchange, as the reference GPU Dock never does: every
+    replica keeps its own temperature."""
+    lines = _run(40, 0, 4)
+    fa, _ = _finals(lines)
+    nl, np_, nt = [int(x) for x in [l for l in lines if l.startswith("sizes ")][0].split()[1:]]
+    tmp = fa[:, 0].astype(int)
+    for p in range(np_):
+        for t in range(nt):
+            for l in range(nl):
+                assert tmp[nt * nl * p + nl * t + l] == t
+
+
+def test_dropin_records_every_dump():
+    """450 steps in dumps of 200: the last dump (steps 400..449) is what rh
+    holds, as after the reference's per-dump copies; replica 0's first record
+    of it is at a step >= 400 or it has none."""
+    lines = _run(450)
+    rec0 = [l for l in lines if l.startswith("  0 ") and "Ref Result" not in l]
+    _, nrec = _finals(lines)
+    assert np.all((nrec >= 0) & (nrec <= 50))
+    if nrec[0] > 0:
+        assert int(rec0[0].split()[1]) >= 400

@@ -31,17 +31,40 @@ def golden_dev(golden_job):
     return g.DeviceJob(golden_job)
 
 
-def _parity(job, dev, oracle, conf, pose, label):
+# Transform bar: each GPU coordinate x = R a + (m + center) within
+# TRANSFORM_ULPS units in the last place of its operands' scale
+# max(|x|, |m + center|, conformer radius) -- an FP32 sum rounds relative to
+# its operands, so a coordinate near 0 A built from 40 A terms is held to the
+# ulp of 40 A, not of its own value.
+TRANSFORM_ULPS = 4
+
+
+def _transform_scale(job, pose, x):
+    a = job.arrays()
+    rad = float(np.sqrt(a["lig_x"] ** 2 + a["lig_y"] ** 2 + a["lig_z"] ** 2).max())
+    t = np.abs(pose[:, None, :3].astype(np.float64) + np.array(job.desc.lig_center[:3], np.float64))
+    s = np.maximum(np.maximum(np.abs(x), t), rad).astype(np.float32)
+    return np.spacing(s).astype(np.float64)
+
+
+def _parity(job, dev, oracle, conf, pose, label, check_transform=True):
     terms, raw, coords = dev.eval(conf, pose, want_coords=True)
-    o_t, o_raw, o_scale, o_c = oracle.energy(job, conf, pose, mode=2, want_coords=True)
-    # transform parity: GPU sincos/FMA vs the oracle's libm transform
-    assert np.abs(coords - o_c).max() < 5e-5, "transform differs"
+    if check_transform:
+        # transform parity: GPU sincos/FMA against the FP64 transform, in ulps
+        _, _, _, o_c = oracle.energy(job, conf, pose, mode=2, want_coords=True)
+        t_ulps = np.abs(coords.astype(np.float64) - o_c) / _transform_scale(job, pose, o_c)
+        print("\n[%s] transform: max %.2f ulp" % (label, t_ulps.max()))
+        assert t_ulps.max() <= TRANSFORM_ULPS, "transform differs by %.1f ulp" % t_ulps.max()
     # energy parity at the GPU coordinates
     r_t, r_raw, r_scale, _ = oracle.energy(job, conf, pose, mode=2, coords=coords)
     err = rel_err(raw, r_raw, r_scale)
     worst = err.max(0)
-    print("\n[%s] max rel err per raw term vs FP64:" % label,
-          " ".join("%s=%.2e" % (n, e) for n, e in zip(g.abi.TERMS[:9], worst)))
+    # the plain relative error |gpu - fp64| / |fp64|, reported beside the bar's
+    # measure; it is the bar itself for the terms whose contributions share a
+    # sign (H-bond, KDE: scale == |value|)
+    plain = (np.abs(raw - r_raw) / np.maximum(np.abs(r_raw), 1e-30)).max(0)
+    print("[%s] max rel err per raw term vs FP64 (scaled | plain):" % label,
+          " ".join("%s=%.2e|%.2e" % (n, e, q) for n, e, q in zip(g.abi.TERMS[:9], worst, plain)))
     assert np.all(np.isfinite(raw)), "non-finite energy"
     assert worst.max() <= TOL, worst
     # the normalised terms and the total follow the same bar on their own scale
@@ -132,3 +155,49 @@ def test_electrostatics_short_range(ene, oracle, el0, el1):
     dev = g.DeviceJob(job)
     conf, pose = random_poses(job, 128, seed=23, trans=1.5)
     _parity(job, dev, oracle, conf, pose, "el0=%g el1=%g" % (el0, el1))
+
+
+@pytest.mark.parametrize("sigma", [4.0, 6.0])
+def test_golden_displaced_poses(golden_job, golden_dev, oracle, sigma):
+    """Poses displaced sigma A (per axis) from the pocket: the band where the
+    H-bond sum is carried by pairs in the tail of its Gaussian."""
+    conf, pose = random_poses(golden_job, max(1000, N_POSES), seed=int(31 + sigma), trans=sigma)
+    _parity(golden_job, golden_dev, oracle, conf, pose, "1a07C1 sigma=%g" % sigma)
+
+
+@pytest.fixture(scope="module")
+def c2_job(ene):
+    # the 1,760-replica benchmark complex: P = 3,000 points x 11, L = 40 atoms x 20, 8 temperatures
+    return g.synth_complex(3000, 40, 20, 11, 8, floor_temp=0.004, ceiling_temp=0.04, seed=2, ene=ene)
+
+
+@pytest.mark.parametrize("sigma", [4.0, 6.0])
+def test_c2_displaced_poses(c2_job, oracle, sigma):
+    dev = g.DeviceJob(c2_job)
+    conf, pose = random_poses(c2_job, max(128, N_POSES // 8), seed=int(41 + sigma), trans=sigma)
+    _parity(c2_job, dev, oracle, conf, pose, "C2 sigma=%g" % sigma)
+
+
+def test_c2_trajectory_poses(c2_job, oracle):
+    """The poses an MC run visits: 2,000 Gaussian-move steps of every replica
+    with exchange on all three axes, then the final accepted pose of 256
+    replicas spread over the ladder.  The terms the run stored for those poses
+    must equal a fresh evaluation bit for bit, and both meet the FP64 bar.
+    Replicas whose forced initial accept underflowed (initial total above
+    104 T, the reference's expf quirk: the four coldest temperatures here)
+    never hold a state, so they are left out."""
+    dev = g.DeviceJob(c2_job)
+    mc = g.make_mc(2000, steps_per_exchange=100, move_mode="gaussian", seed=7,
+                   axes=("temp", "lig", "prt"))
+    out, _ = dev.dock(mc)
+    d = c2_job.desc
+    live = np.flatnonzero((out.n_accept > 0) | (out.n_swap_lig > 0) | (out.n_swap_prt > 0))
+    assert live.size >= 256
+    reps = live[np.linspace(0, live.size - 1, 256).astype(np.int64)]
+    conf = np.stack([reps % d.n_lig, reps // (d.n_tmp * d.n_lig)], 1).astype(np.int32)
+    pose = np.ascontiguousarray(out.movematrix[reps], np.float32)
+    moved = np.linalg.norm(pose[:, :3], axis=1)
+    print("\n[C2 trajectory] displacement from the pocket: median %.2f A, max %.2f A"
+          % (np.median(moved), moved.max()))
+    terms, _ = _parity(c2_job, dev, oracle, conf, pose, "C2 after 2000 MC steps")
+    assert np.array_equal(terms, out.energy[reps]), "stored terms differ from a fresh evaluation"
