@@ -16,7 +16,19 @@ the dominant kernel (gd_pair_kernel<LC>, LC ligand atoms per lane): algorithmic 
 its mean launch time from HIP events recorded on the job's stream around
 the launch of every 16th MC step (625 launches over the job; the library
 samples so that the records, about 4 us each, stay under 1% of the run).  cpu_baseline times the CPU oracle (a restatement of the
-reference CPU backend) on a bounded sample of the same workload.
+reference CPU backend, in the reference's FP32 expression order) on a bounded sample of the same workload.
+
+The same line carries, unless --no-extra:
+  reference_shape  the headline job in the reference's plain shape (dumps of
+                   200 steps, a blocking copy of each dump's records);
+  c1               the CPU reference configuration in full (GPU, oracle and
+                   the reference CPU backend itself where oracle/_ref is built);
+  c3               the batched screen in full (256 ligands x 440 replicas x
+                   5,000 steps against one 4,000-point target) against the plain
+                   path in both shapes on 32 randomly drawn jobs, with its own
+                   roofline and CPU baseline.
+--workload c1|c3|c4|c5 print one of these alone (c4: heterogeneous batch,
+c5: Poisson serving at 50/200/800 arrivals/s).
 """
 import argparse
 import ctypes as C
@@ -87,9 +99,21 @@ def cpu_model():
     return name
 
 
+_HOST_CORES = None
+
+
 def host_cores():
     """(logical CPUs this process may run on, physical cores among them,
-    physical cores of the whole host)."""
+    physical cores of the whole host), read once: an OpenMP run with
+    OMP_PROC_BIND pins the calling thread, after which its affinity mask
+    shows one CPU."""
+    global _HOST_CORES
+    if _HOST_CORES is None:
+        _HOST_CORES = _read_host_cores()
+    return _HOST_CORES
+
+
+def _read_host_cores():
     try:
         allowed = sorted(os.sched_getaffinity(0))
     except AttributeError:
@@ -128,54 +152,101 @@ def cpu_fields(threads):
             "threads_note": "one OpenMP thread per physical core, at most 16 (a one-GPU job's CPU share)"}
 
 
-def cpu_baseline(job, mc_template, budget_s=15.0):
-    """Time the CPU oracle on all replicas for a bounded number of MC steps."""
-    threads = cpu_threads()
+def oracle_rate(job, steps, threads, rep_hi=0):
+    """replica-steps/s of the CPU oracle in the reference's own FP32 expression
+    order (OpenMP over replicas) for `steps` MC steps of replicas [0, rep_hi)
+    (0: all); the timed region is the MC loop, initial pass included."""
     from tests.oracle_ffi import Oracle
     import geauxdock_amd as g
     orc = Oracle()
-    # calibrate on one step, then run a sample of about budget_s seconds
-    mc1 = g.make_mc(1, move_mode="gaussian", seed=SEED)
-    t0 = time.perf_counter()
-    orc.dock(job, mc1, threads=threads)
-    t1 = time.perf_counter() - t0          # initial pass + 1 step
-    per_step = max(t1 / 2.0, 1e-3)
-    steps = max(1, min(200, int(budget_s / per_step)))
     mc = g.make_mc(steps, move_mode="gaussian", seed=SEED)
-    _, _, ms = orc.dock(job, mc, threads=threads)
-    # ms covers the initial pass + `steps` steps
-    rs = job.n_rep * (steps + 1) / (ms / 1000.0)
+    _, _, ms = orc.dock(job, mc, threads=threads, rep_hi=rep_hi, literal=True)
+    n = rep_hi if rep_hi > 0 else job.n_rep
+    return n * (steps + 1) / (ms / 1000.0), ms
+
+
+def cpu_baseline(job, budget_s=15.0, note=""):
+    """The CPU oracle on all replicas of `job` for a bounded number of MC steps
+    (about budget_s seconds, calibrated on one step)."""
+    threads = cpu_threads()
+    hw = cpu_fields(threads)   # before OpenMP binds this thread to one core
+    _, ms1 = oracle_rate(job, 1, threads)          # initial pass + 1 step
+    per_step = max(ms1 / 2000.0, 1e-3)
+    steps = max(1, min(200, int(budget_s / per_step)))
+    rs, _ = oracle_rate(job, steps, threads)
     out = {"value": rs, "unit": "replica-steps/s", "kind": "port",
-           "sample": "all %d replicas x %d MC steps (+ initial pass) of the same complex, "
-                     "OpenMP over replicas; the reference backend's fixed capacities (2048 protein "
-                     "points) cannot hold this complex, so its restatement is timed" % (job.n_rep, steps)}
-    out.update(cpu_fields(threads))
+           "sample": "all %d replicas x %d MC steps (+ initial pass) of the same complex, OpenMP over "
+                     "replicas, the reference's FP32 expression order%s" % (job.n_rep, steps, note)}
+    out.update(hw)
     return out
 
 
-def plain_c1():
-    """The CPU reference configuration in full: P=1200 points x 5 conformers,
-    L=24 atoms x 5 conformers, 1 temperature (25 replicas), 3000 MC steps,
-    Gaussian moves N(0, 0.2 A) / N(0, 3 deg), seed 1; the CPU oracle (OpenMP over
-    replicas) and the plain GPU path on the same job and draws."""
+def timed_plain(host, mc, reference_shape):
+    """One job, one job per launch sequence, timed from the host complex to its
+    results: device-job creation (packing + H2D), the run and the release.
+    reference_shape: records on, a blocking copy of every dump's records to the
+    host after the dump (backend_gpu.cu:96-118), as the reference's plain path."""
+    import geauxdock_amd as g
+    t0 = time.perf_counter()
+    d = g.DeviceJob(host)
+    if reference_shape:
+        d.dock_stream(mc, lambda *a: None, blocking=True, stats=False)
+    else:
+        d.dock(mc, stats=False)
+    d.close()
+    return time.perf_counter() - t0
+
+
+def roofline(flops, ms, kernel, **extra):
+    achieved = flops / (ms / 1000.0) / 1e12
+    r = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+         "frac": achieved / PEAK_FP32_TFLOPS, "kernel": kernel, "flops_per_launch": flops,
+         "mean_launch_ms": ms}
+    r.update(extra)
+    return r
+
+
+def c1_extra():
+    """The CPU reference configuration in full: P = 1,200 points x 5 conformers,
+    L = 24 atoms x 5 conformers, 1 temperature (25 replicas), 3,000 MC steps,
+    Gaussian moves N(0, 0.2 A) / N(0, 3 deg), seed 1: the plain GPU path (its
+    own shape and the reference's), the CPU oracle and, where it was built,
+    the reference CPU backend itself."""
     import geauxdock_amd as g
     threads = cpu_threads()
-    from tests.oracle_ffi import Oracle
+    hw = cpu_fields(threads)
     job = g.synth_complex(1200, 24, 5, 5, 1, floor_temp=0.044, ceiling_temp=0.044, seed=1)
     mc = g.make_mc(3000, steps_per_dump=3000, move_mode="gaussian", seed=1)
+    mc_ref = g.make_mc(3000, steps_per_dump=200, move_mode="gaussian", seed=1)
+    flops, pairs = algorithmic_flops_per_step(job)
     dev = g.DeviceJob(job)
     dev.dock(g.make_mc(100, move_mode="gaussian", seed=1), stats=False)   # warm-up
     t0 = time.perf_counter()
     out, st = dev.dock(mc, stats=True)
     tg = time.perf_counter() - t0
-    orc = Oracle()
-    ref, _, ms = orc.dock(job, mc, threads=threads)
+    lc = dev.tiling()[0]
+    dev.close()
+    t_ref = timed_plain(job, mc_ref, True)
+    pair_ms = st.ms_pair_kernel / max(1, st.n_pair_launch)
     rs_gpu = job.n_rep * 3000 / tg
-    rs_cpu = job.n_rep * 3001 / (ms / 1000.0)
-    pa, pb = out.n_accept.mean() / 3000, ref.n_accept.mean() / 3000
-    # the reference CPU backend itself (compiled from its sources into
-    # oracle/_ref, where that was built), same job, its own moves and RNG
-    ref_leg = None
+    rs_cpu, cpu_ms = oracle_rate(job, 3000, threads)
+    line = {"workload": "C1 in full: P=1200 points x 5 conformers, L=24 atoms x 5 conformers, 1 "
+                        "temperature (25 replicas), 3000 MC steps, plain GPU path",
+            "value": rs_gpu, "unit": "replica-steps/s", "ms": tg * 1000.0,
+            "jobs_per_s": 1.0 / tg,
+            "reference_shape": {"jobs_per_s": 1.0 / t_ref, "replica_steps_per_s": job.n_rep * 3000 / t_ref,
+                                "shape": "steps_per_dump 200, blocking record copy after each dump; "
+                                         "timed from the host complex to results"},
+            "roofline": roofline(flops, pair_ms, "gd_pair_kernel<%d>" % lc,
+                                 launches_timed=st.n_pair_launch, prt_pairs_per_launch=pairs,
+                                 note="25 replicas are one 64-lane block: the launch fills a "
+                                      "few CUs of 256"),
+            "acceptance_gpu": float(out.n_accept.mean() / 3000)}
+    cpu = {"value": rs_cpu, "unit": "replica-steps/s", "kind": "port", "ms": cpu_ms,
+           "sample": "the whole job (25 replicas x 3000 MC steps + initial pass), OpenMP over "
+                     "replicas, the reference's FP32 expression order"}
+    cpu.update(hw)
+    # the reference CPU backend itself, compiled from its sources into oracle/_ref
     lib = os.path.join(ROOT, "oracle", "_ref", "libgdref.so")
     if os.path.exists(lib):
         from geauxdock_amd import abi
@@ -185,19 +256,14 @@ def plain_c1():
         rms = C.c_double()
         scale = (C.c_float * 6)(*[float(x) for x in mc.move_scale])
         if R.ref_dock(C.byref(job.desc), 3000, 200, scale, C.byref(rms), None) == 0:
-            ref_leg = {"replica_steps_per_s": job.n_rep * 3001 / (rms.value / 1000.0), "ms": rms.value,
-                       "threads": threads, "kind": "reference",
-                       "note": "backend_cpu_mic Dock compiled from the reference sources, uniform moves "
-                               "of the same scale, its own RNG"}
-    return {"workload": "c1", "replicas": job.n_rep, "mc_steps": 3000, "cpu_reference": ref_leg,
-            "gpu_replica_steps_per_s": rs_gpu, "gpu_ms": tg * 1000.0,
-            "cpu_replica_steps_per_s": rs_cpu, "cpu_ms": ms, "cpu_threads": threads,
-            "cpu_model": cpu_model(), "gpu_over_cpu": rs_gpu / rs_cpu,
-            "acceptance_gpu": pa, "acceptance_cpu": pb,
-            "best_energy_mean_gpu": float(out.best_energy.mean()),
-            "best_energy_mean_cpu": float(ref.best_energy.mean()),
-            "note": "25 replicas fill a fraction of either device; the CPU leg is the whole job, "
-                    "MC loop only (initial pass included)"}
+            cpu = {"value": job.n_rep * 3001 / (rms.value / 1000.0), "unit": "replica-steps/s",
+                   "kind": "reference", "ms": rms.value,
+                   "sample": "the whole job: backend_cpu_mic Dock compiled from the reference's "
+                             "sources (oracle/_ref), uniform moves of the same scales, its own RNG",
+                   "port_value": rs_cpu}
+            cpu.update(hw)
+    line["cpu_baseline"] = cpu
+    return line
 
 
 def _sizes(rng, n, kind):
@@ -206,32 +272,44 @@ def _sizes(rng, n, kind):
     for i in range(n):
         L = int(rng.integers(12, 81))
         if kind == "c3":
-            out.append(dict(P=4000, L=L, nlig=10, nprt=11, ntmp=4))
+            out.append(dict(P=4000, L=L, nlig=10, nprt=11, ntmp=4, prot=3))
         else:
             P = int(round(np.exp(rng.uniform(np.log(1500), np.log(9000)))))
             nlig = 5 if kind == "c4" else int(rng.integers(5, 21))
-            out.append(dict(P=P, L=L, nlig=nlig, nprt=11, ntmp=4))
+            out.append(dict(P=P, L=L, nlig=nlig, nprt=11, ntmp=4, prot=1000 + i // 32))
     return out
 
 
-def batched(kind, n_jobs, mc_steps, batch_size=0):
+def _hosts(kind, sizes, seed0):
+    import geauxdock_amd as g
+    hosts = []
+    for i, z in enumerate(sizes):
+        if kind == "c5":
+            hosts.append(g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.004,
+                                         ceiling_temp=0.04, seed=seed0 + i))
+        else:   # C3: one target for every ligand; C4: 32 ligands per target
+            hosts.append(g.synth_screen(z["P"], z["prot"], z["L"], seed0 + i, z["nlig"], z["nprt"],
+                                        z["ntmp"], floor_temp=0.004, ceiling_temp=0.04))
+        if (i + 1) % 256 == 0:
+            print("[%s] %d complexes generated" % (kind, i + 1), file=sys.stderr, flush=True)
+    return hosts
+
+
+def batched(kind, n_jobs, mc_steps, batch_size=0, plain_sample=0, roofline_steps=128, cpu=True):
     """Batched form (one launch sequence, persistent work-queue pair kernel,
-    graph-captured segments) vs the plain one-job-per-launch path on the
-    same complexes; jobs/s of each.  Both legs are timed from the host
-    complexes to the results: device-job creation (packing + H2D), the run and
-    the release are inside each leg's clock, and the plain leg runs every job.
-    batch_size > 0 runs the batched leg in consecutive batches of that many
-    jobs (a screen larger than one batch should hold)."""
+    graph-captured segments, tilings from the tuned table) against the plain
+    one-job-per-launch path on the same complexes: jobs/s of each, both timed
+    from the host complexes to the results.  The plain path runs in its own
+    shape and in the reference's (per-dump blocking record copies), on every
+    job or, with plain_sample > 0, on a seeded sample of that many jobs (jobs/s
+    = sample / its summed time).  The roofline comes from a separate timed run
+    of roofline_steps steps of the whole batch (HIP events on the persistent
+    pair kernel's launches)."""
     import geauxdock_amd as g
     from geauxdock_amd import abi
     rng = np.random.default_rng(3 if kind == "c3" else 4)
     sizes = _sizes(rng, n_jobs, kind)
-    hosts = []
-    for i, z in enumerate(sizes):
-        hosts.append(g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.004,
-                                     ceiling_temp=0.04, seed=100 + i))
-        if (i + 1) % 256 == 0:
-            print("[%s] %d complexes generated" % (kind, i + 1), file=sys.stderr, flush=True)
+    hosts = _hosts(kind, sizes, 100)
     mcs = [g.make_mc(mc_steps, steps_per_exchange=100, move_mode="gaussian", seed=5, job_id=i,
                      axes=AXES) for i in range(n_jobs)]
     mca = (abi.gd_mc_param * n_jobs)(*mcs)
@@ -247,8 +325,6 @@ def batched(kind, n_jobs, mc_steps, batch_size=0):
         hi = min(n_jobs, lo + bs)
         ta = time.perf_counter()
         devs = [g.DeviceJob(h) for h in hosts[lo:hi]]
-        for d in devs:                   # shared-launch tiling: 4 atoms x 64 points
-            d.set_tiling(4, min(64, d.host.desc.prt_npoint))
         arr = (C.c_void_p * (hi - lo))(*[d.h.value for d in devs])
         tb_ = time.perf_counter()
         mc_lo = C.cast(C.addressof(mca) + lo * C.sizeof(abi.gd_mc_param), C.POINTER(abi.gd_mc_param))
@@ -263,44 +339,87 @@ def batched(kind, n_jobs, mc_steps, batch_size=0):
               file=sys.stderr, flush=True)
     tb = time.perf_counter() - t0
     rep_steps = sum(h.n_rep for h in hosts) * mc_steps
-    # plain path: every complex one at a time, default tiling
-    t0 = time.perf_counter()
-    for i in range(n_jobs):
-        dp = g.DeviceJob(hosts[i])
-        dp.dock(mcs[i], stats=False)
-        dp.close()
-        if (i + 1) % 32 == 0:
-            print("[%s] plain leg: %d jobs in %.1f s" % (kind, i + 1, time.perf_counter() - t0),
-                  file=sys.stderr, flush=True)
-    tp = time.perf_counter() - t0
-    # each compared job's best energies against its own plain run (same tiling)
+    pair_evals = sum(h.n_rep * h.desc.lig_natom * h.desc.prt_npoint for h in hosts) * (mc_steps + 1)
+    # roofline: the persistent pair kernel of the first batch, event-timed
+    nb = min(n_jobs, bs)
+    devs = [g.DeviceJob(h) for h in hosts[:nb]]
+    arr = (C.c_void_p * nb)(*[d.h.value for d in devs])
+    rmc = (abi.gd_mc_param * nb)(*[g.make_mc(roofline_steps, steps_per_exchange=100, seed=5, job_id=i,
+                                             axes=AXES) for i in range(nb)])
+    st = abi.gd_stats()
+    g.check(g.lib().gd_batch_dock(arr, nb, rmc, None, C.byref(st)), "gd_batch_dock (timed)")
+    flops = sum(algorithmic_flops_per_step(h)[0] for h in hosts[:nb])
+    pairs = sum(algorithmic_flops_per_step(h)[1] for h in hosts[:nb])
+    tiling = sorted({d.tiling() for d in devs})
+    for d in devs:
+        d.close()
+    pair_ms = st.ms_pair_kernel / max(1, st.n_pair_launch)
+    # plain legs
+    which = list(range(n_jobs))
+    if 0 < plain_sample < n_jobs:
+        which = sorted(np.random.default_rng(11).choice(n_jobs, plain_sample, replace=False).tolist())
+    tp = tr = 0.0
+    for q, i in enumerate(which):
+        tp += timed_plain(hosts[i], mcs[i], False)
+        mc_r = g.make_mc(mc_steps, steps_per_dump=200, steps_per_exchange=100, move_mode="gaussian",
+                         seed=5, job_id=i, axes=AXES)
+        tr += timed_plain(hosts[i], mc_r, True)
+        if (q + 1) % 16 == 0:
+            print("[%s] plain legs: %d jobs in %.1f s" % (kind, q + 1, tp + tr), file=sys.stderr, flush=True)
+    plain_jps, ref_jps = len(which) / tp, len(which) / tr
+    # each sampled job's bits against its own plain run at the shared tiling
     same = []
-    for i in range(min(2, n_jobs)):
+    for i in which[:2]:
         dp = g.DeviceJob(hosts[i])
-        dp.set_tiling(4, min(64, hosts[i].desc.prt_npoint))
+        lc, pt, _ = g.host.tiling_lookup(hosts[i].desc.lig_natom, hosts[i].desc.prt_npoint,
+                                         hosts[i].n_rep, True)
+        dp.set_tiling(lc, pt)
         o, _ = dp.dock(mcs[i], stats=False)
         dp.close()
         same.append(bool(np.array_equal(o.best_energy, outs[i].best_energy)))
-    return {"workload": kind, "jobs": n_jobs, "mc_steps": mc_steps,
-            "batched_jobs_per_s": n_jobs / tb, "batched_replica_steps_per_s": rep_steps / tb,
-            "plain_jobs_per_s": n_jobs / tp,
-            "speedup_jobs_per_s": tp / tb, "timed": "host complexes to results, both legs",
+    line = {"workload": kind, "jobs": n_jobs, "mc_steps": mc_steps,
+            "replicas": int(sum(h.n_rep for h in hosts)),
+            "value": n_jobs / tb, "unit": "jobs/s",
+            "replica_steps_per_s": rep_steps / tb, "pair_evals_per_s": pair_evals / tb,
+            "timed": "host complexes to results (device-job creation, run, release), both legs",
             "batched_setup_s": t_setup, "batched_run_s": t_run, "batch_size": bs,
-            "best_energy_bitwise_equal_to_plain": same}
+            "tilings": [list(t) for t in tiling],
+            "plain_jobs_per_s": plain_jps, "plain_reference_shape_jobs_per_s": ref_jps,
+            "plain_jobs_timed": len(which),
+            "plain_sample": ("all jobs" if len(which) == n_jobs else
+                             "%d of %d jobs drawn at random (seed 11), each timed alone" % (len(which), n_jobs)),
+            "ratio_vs_plain": (n_jobs / tb) / plain_jps,
+            "ratio_vs_plain_reference_shape": (n_jobs / tb) / ref_jps,
+            "target_10x_met": (n_jobs / tb) / ref_jps >= 10.0,
+            "best_energy_bitwise_equal_to_plain": same,
+            "roofline": roofline(flops, pair_ms, "gd_pair_kernel_wq<4>", launches_timed=st.n_pair_launch,
+                                 prt_pairs_per_launch=pairs,
+                                 measured="%d MC steps of the first %d jobs, every 16th pair launch "
+                                          "event-timed" % (roofline_steps, nb))}
+    if cpu:
+        rs, ms = oracle_rate(hosts[0], 20, cpu_threads())
+        cb = {"value": rs, "unit": "replica-steps/s", "kind": "port",
+              "jobs_per_s_extrapolated": rs / (hosts[0].n_rep * mc_steps) *
+              (hosts[0].n_rep * hosts[0].desc.lig_natom / np.mean([h.n_rep * h.desc.lig_natom for h in hosts])),
+              "sample": "job 0 (%d replicas, L=%d) for 20 MC steps + initial pass, OpenMP over replicas, "
+                        "the reference's FP32 expression order; jobs/s scaled by the mean job's "
+                        "replicas x ligand atoms" % (hosts[0].n_rep, hosts[0].desc.lig_natom)}
+        cb.update(cpu_fields(cpu_threads()))
+        line["cpu_baseline"] = cb
+    return line
 
 
 def serve(rates, n_jobs, step_scale):
     """Serving: Poisson arrivals, admission by device footprint, continuous
     batching; p50/p99 latency (arrival -> result) and jobs/s per rate, and the
-    plain path's jobs/s on the same requests."""
+    plain path's jobs/s on the same requests (both shapes)."""
     import geauxdock_amd as g
     from geauxdock_amd import abi
     res = []
     rng = np.random.default_rng(5)
     sizes = _sizes(rng, n_jobs, "c5")
     steps = [int(rng.integers(2000, 10001) * step_scale) for _ in range(n_jobs)]
-    hosts = [g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.004,
-                             ceiling_temp=0.04, seed=500 + i) for i, z in enumerate(sizes)]
+    hosts = _hosts("c5", sizes, 500)
     mcs = [g.make_mc(steps[i], steps_per_exchange=100, move_mode="gaussian", seed=7, job_id=i,
                      axes=AXES) for i in range(n_jobs)]
     for lam in rates:
@@ -339,17 +458,16 @@ def serve(rates, n_jobs, step_scale):
         res.append({"rate_per_s": lam, "jobs": n_jobs, "jobs_per_s": n_jobs / wall,
                     "p50_latency_s": float(np.percentile(lat, 50)),
                     "p99_latency_s": float(np.percentile(lat, 99))})
-    t0 = time.perf_counter()
+    tp = tr = 0.0
     for i in range(n_jobs):
-        d = g.DeviceJob(hosts[i])
-        d.dock(mcs[i], stats=False)
-        d.close()
+        tp += timed_plain(hosts[i], mcs[i], False)
+        mc_r = g.make_mc(steps[i], steps_per_dump=200, steps_per_exchange=100, move_mode="gaussian",
+                         seed=7, job_id=i, axes=AXES)
+        tr += timed_plain(hosts[i], mc_r, True)
         if (i + 1) % 50 == 0:
-            print("[c5] plain: %d jobs in %.1f s" % (i + 1, time.perf_counter() - t0), file=sys.stderr,
-                  flush=True)
-    tp = time.perf_counter() - t0
+            print("[c5] plain: %d jobs in %.1f s" % (i + 1, tp + tr), file=sys.stderr, flush=True)
     return {"workload": "c5-serving", "step_scale": step_scale, "per_rate": res,
-            "plain_jobs_per_s": n_jobs / tp,
+            "plain_jobs_per_s": n_jobs / tp, "plain_reference_shape_jobs_per_s": n_jobs / tr,
             "plain_note": "same requests, one job per launch sequence, back to back"}
 
 
@@ -420,11 +538,17 @@ def main():
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                     help="summarise rocprofv3 FETCH_SIZE/WRITE_SIZE runs into "
                          "profiles/pair_kernel_c2_traffic.json (no GPU needed)")
-    ap.add_argument("--jobs", type=int, default=32)
-    ap.add_argument("--mc-steps", type=int, default=500)
+    ap.add_argument("--jobs", type=int, default=256, help="c3/c4/c5: jobs (requests)")
+    ap.add_argument("--mc-steps", type=int, default=5000, help="c3/c4: MC steps per job; c5: "
+                    "the step scale is mc_steps / 10000 of the 2,000-10,000 drawn per request")
     ap.add_argument("--batch-size", type=int, default=0,
                     help="c3/c4: jobs per batched launch sequence (0: all jobs in one)")
+    ap.add_argument("--plain-sample", type=int, default=32,
+                    help="c3/c4: plain legs on this many randomly drawn jobs (0: every job)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default run: the headline line only, without its c1/c3 keys")
     args = ap.parse_args()
+    host_cores()   # the CPU topology, before any OpenMP region binds this thread
     if args.traffic:
         print(json.dumps(traffic_summary(*args.traffic, os.path.join(ROOT, "profiles",
                                                                       "pair_kernel_c2_traffic.json"))))
@@ -441,11 +565,11 @@ def main():
     from geauxdock_amd import abi
     g.check(g.lib().gd_init(0), "gd_init")
     if args.workload == "c1":
-        print(json.dumps(plain_c1()), flush=True)
+        print(json.dumps(c1_extra()), flush=True)
         return
     if args.workload in ("c3", "c4"):
-        print(json.dumps(batched(args.workload, args.jobs, args.mc_steps, args.batch_size)),
-              flush=True)
+        print(json.dumps(batched(args.workload, args.jobs, args.mc_steps, args.batch_size,
+                                 args.plain_sample, cpu=not args.no_cpu)), flush=True)
         return
     if args.workload == "c5":
         print(json.dumps(serve([50, 200, 800], args.jobs, args.mc_steps / 10000.0)), flush=True)
@@ -527,10 +651,30 @@ def main():
             "prt_only_tflops": PRT_FLOPS * prt_pairs / (pair_ms / 1000.0) / 1e12,
         },
     }
+    # the same job in the reference's plain shape: dumps of 200 steps, each
+    # followed by a blocking copy of its accept records to the host
+    mc_ref = g.make_mc(steps, steps_per_dump=200, steps_per_exchange=STEPS_PER_EXCHANGE,
+                       move_mode="gaussian", seed=SEED, axes=AXES)
+    t0 = time.perf_counter()
+    dev.dock_stream(mc_ref, lambda *a: None, blocking=True, stats=False)
+    wall_ref = time.perf_counter() - t0
+    line["reference_shape"] = {
+        "value": job.n_rep * steps / wall_ref, "unit": "replica-steps/s",
+        "jobs_per_s": (steps / float(C2_JOB_STEPS)) / wall_ref,
+        "shape": "the same job and steps, dumps of 200 steps each followed by a blocking copy of "
+                 "the dump's accept records to the host (backend_gpu.cu:96-118)"}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(job, mc)
+        line["cpu_baseline"] = cpu_baseline(job, note="; the reference backend's fixed capacities "
+                                            "(2048 protein points) cannot hold this complex, so its "
+                                            "restatement is timed")
     else:
         line["cpu_baseline"] = None
+    dev.close()
+    if not args.no_extra:
+        print("[bench] headline done; C1 in full", file=sys.stderr, flush=True)
+        line["c1"] = c1_extra()
+        print("[bench] C1 done; C3 (256 ligands x 5000 steps, batched)", file=sys.stderr, flush=True)
+        line["c3"] = batched("c3", 256, 5000, 0, 32, cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

@@ -39,5 +39,5 @@ def check(status, what=""):
                                             msg.decode() if msg else ""))
 
 
-from .host import (HostJob, load_enepara, load_complex, synth_complex, DeviceJob,  # noqa: E402
+from .host import (HostJob, load_enepara, load_complex, synth_complex, synth_screen, DeviceJob,  # noqa: E402
                    make_mc)

@@ -83,7 +83,8 @@ class gd_tiling(C.Structure):
 
 
 class gd_server_config(C.Structure):
-    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("sweep_steps", C.c_int)]
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("sweep_steps", C.c_int),
+                ("converge_sweeps", C.c_int), ("converge_tol", C.c_float)]
 
 
 class gd_request_files(C.Structure):
@@ -134,6 +135,9 @@ PROTOTYPES = {
     "gd_tiling_get": (C.c_int, [_vp, C.POINTER(gd_tiling)]),
     "gd_tiling_set": (C.c_int, [_vp, C.POINTER(gd_tiling)]),
     "gd_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(gd_tiling), C.POINTER(C.c_double)]),
+    "gd_autotune_shared": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.POINTER(gd_tiling),
+                                     C.POINTER(C.c_double)]),
+    "gd_tiling_lookup": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(gd_tiling)]),
     "gd_server_create": (C.c_int, [C.POINTER(gd_server_config), C.POINTER(_vp)]),
     "gd_server_destroy": (C.c_int, [_vp]),
     "gd_server_submit": (C.c_int, [_vp, C.POINTER(gd_job_desc), C.POINTER(gd_mc_param),
@@ -153,6 +157,8 @@ PROTOTYPES = {
                                   C.POINTER(gd_enepara), C.c_int, _f, _f, C.POINTER(_vp)]),
     "gd_synth_complex": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f, _f,
                                    C.c_uint64, C.POINTER(gd_enepara), C.POINTER(_vp)]),
+    "gd_synth_screen": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int,
+                                  _f, _f, C.POINTER(gd_enepara), C.POINTER(_vp)]),
     "gd_host_job_desc": (C.c_int, [_vp, C.POINTER(gd_job_desc)]),
     "gd_host_job_free": (C.c_int, [_vp]),
     "gd_kabsch_rmsd": (_f, [C.c_int, _fp, _fp]),

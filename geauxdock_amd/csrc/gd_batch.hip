@@ -19,6 +19,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -128,16 +129,20 @@ int pair_grid(int waves_per_cu) {  // work groups of GD_GROUP waves
   return std::max(1, cu * std::max(GD_GROUP, waves_per_cu) / GD_GROUP);
 }
 
-// Resident waves per CU of the persistent pair kernel: as many 4-wave groups as
-// the kernel's registers allow (4 atoms: ~60 VGPRs, 8 waves per SIMD).
-int pair_waves_per_cu(int lc) { return lc == 4 ? 32 : lc == 8 ? 20 : 8; }
-
-// Tiling of a job inside shared launches (batches, serving): a function of the
-// complex only, so a job's bits do not depend on what it runs with.  Tiles of
-// 64 points keep the per-item fixed cost small; the launch holds items of many
-// jobs, so the machine is full without small tiles.
+// Jobs in shared launches (batches, serving) run 4 atoms per lane and the
+// table's tile size for their size bucket (shared_ptile): a function of the
+// complex only, so a job's bits do not depend on what it runs with.
 constexpr int kSharedLc = 4;
-int shared_ptile(int P) { return std::min(P, 64); }
+
+// Bring a job to its shared-launch tiling (ptile_override > 0: that tile size
+// instead, for the sweeps of gd_autotune_shared).
+gd_status shared_tiling(gd_job *j, int ptile_override) {
+  const int pt = ptile_override > 0 ? std::min(j->h.P, ptile_override) : shared_ptile(j->h.L, j->h.P);
+  if (j->h.lc == kSharedLc && j->h.ptile == pt) return GD_OK;
+  gd_status st = apply_tiling(j, kSharedLc, pt);
+  if (st != GD_OK) return st;
+  return sync_job_struct(j);
+}
 
 // Work of one exchange point: temperature-exchange entries and, per job of the
 // lists, the round of its conformer swaps (-1 = none).
@@ -229,35 +234,19 @@ gd_status batch_init(const BatchLists &B, int lc, int grid, hipStream_t s) {
   return GD_OK;
 }
 
-// Bring every job to one ligand-chunk width (one kernel template per launch).
-gd_status common_lc(const std::vector<gd_job *> &jobs, int &lc) {
-  lc = jobs.empty() ? kSharedLc : jobs[0]->h.lc;
-  bool same = true;
-  for (gd_job *j : jobs) same = same && j->h.lc == lc;
-  if (same) return GD_OK;
-  lc = kSharedLc;
-  for (gd_job *j : jobs)
-    if (j->h.lc != lc) {
-      gd_status st = apply_tiling(j, lc, j->h.ptile);
-      if (st != GD_OK) return st;
-      if ((st = sync_job_struct(j)) != GD_OK) return st;
-    }
-  return GD_OK;
-}
-
-}  // namespace
-
-// ---------------------------------------------------------------------------
-extern "C" gd_status gd_batch_dock(gd_job **jobs_in, int n_jobs, const gd_mc_param *mc,
-                                   gd_replica_out *out, gd_stats *stats) {
+// Batched docking (gd_batch_dock); the overrides (> 0) replace the shared
+// tile size and the persistent grid's waves per CU (autotuner sweeps).
+gd_status batch_dock_impl(gd_job **jobs_in, int n_jobs, const gd_mc_param *mc, gd_replica_out *out,
+                          gd_stats *stats, int ptile_override, int waves_override) {
   if (!jobs_in || n_jobs <= 0 || !mc) return GD_ERR_ARG;
   gd_status st = ensure_device();
   if (st != GD_OK) return st;
   std::vector<gd_job *> jobs(jobs_in, jobs_in + n_jobs);
   for (gd_job *j : jobs)
     if (!j) return GD_ERR_ARG;
-  int lc;
-  if ((st = common_lc(jobs, lc)) != GD_OK) return st;
+  const int lc = kSharedLc;
+  for (gd_job *j : jobs)
+    if ((st = shared_tiling(j, ptile_override)) != GD_OK) return st;
   hipStream_t s = jobs[0]->stream;
   int steps_max = 0;
   for (int k = 0; k < n_jobs; ++k) {
@@ -274,7 +263,7 @@ extern "C" gd_status gd_batch_dock(gd_job **jobs_in, int n_jobs, const gd_mc_par
   for (int k = 0; k < n_jobs; ++k) active[k] = k;
   BatchLists B;
   if ((st = B.build(jobs, active, s)) != GD_OK) return st;
-  const int grid = pair_grid(pair_waves_per_cu(lc));
+  const int grid = pair_grid(waves_override > 0 ? waves_override : pair_waves_per_cu(lc));
   KTimer tp;
   const auto t0 = std::chrono::steady_clock::now();
   if ((st = batch_init(B, lc, grid, s)) != GD_OK) return st;
@@ -389,50 +378,21 @@ extern "C" gd_status gd_batch_dock(gd_job **jobs_in, int n_jobs, const gd_mc_par
   return GD_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Autotuner: sweep ligand atoms per work item x protein tile size on the
-// job itself for `steps` MC steps each, keep the fastest (HIP-event time of
-// the pair kernel per pair evaluation), cache it by size bucket.
-namespace {
-std::mutex g_tune_mu;
-std::map<long long, std::pair<int, int>> g_tune_cache;  // bucket -> (lc, ptile)
-long long tune_bucket(int L, int P, int R) {
-  auto lg = [](int x) {
-    int b = 0;
-    while ((1 << (b + 1)) <= x) ++b;
-    return b;
-  };
-  return ((long long)lg(L) << 40) | ((long long)lg(P) << 20) | lg(std::max(1, R));
-}
 }  // namespace
 
-namespace gdr {
-bool tuned_tiling(int L, int P, int R, int &lc, int &ptile) {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  auto it = g_tune_cache.find(tune_bucket(L, P, R));
-  if (it == g_tune_cache.end()) return false;
-  lc = it->second.first;
-  ptile = std::min(P, it->second.second);
-  return true;
+extern "C" gd_status gd_batch_dock(gd_job **jobs, int n_jobs, const gd_mc_param *mc,
+                                   gd_replica_out *out, gd_stats *stats) {
+  return batch_dock_impl(jobs, n_jobs, mc, out, stats, 0, 0);
 }
-}  // namespace gdr
 
-extern "C" gd_status gd_autotune(gd_job *j, int steps, gd_tiling *best, double *best_ms) {
-  if (!j || steps <= 0) return GD_ERR_ARG;
-  gd_status st = ensure_device();
-  if (st != GD_OK) return st;
-  const GdJob g0 = j->h;
-  const int L = g0.L, P = g0.P;
-  std::vector<std::pair<int, int>> cand;
-  for (int lc : {4, 8}) {
-    for (int pt : {13, 20, 26, 39, 52, 78, 104, 156})
-      if (pt <= P) cand.push_back({lc, pt});
-  }
-  {
-    int dlc, dpt;
-    default_tiling(L, P, g0.R, dlc, dpt);
-    cand.push_back({dlc, dpt});
-  }
+// ---------------------------------------------------------------------------
+// Autotuner (the tiling table of gd_tiling_table.h is written from these sweeps
+// by geauxdock_amd/tune.py).  The reference's only counterpart is the disabled
+// CalcThreadBlockShape (backend_gpu/kernel_cuda_l2_util.cu:40-67), which picks
+// a block shape by a formula.  Neither sweep touches process-wide state: a
+// job's tiling elsewhere stays a function of the job.
+namespace {
+gd_mc_param tune_mc(int steps) {
   gd_mc_param mc;
   std::memset(&mc, 0, sizeof mc);
   mc.steps_total = steps;
@@ -440,6 +400,33 @@ extern "C" gd_status gd_autotune(gd_job *j, int steps, gd_tiling *best, double *
   mc.move_mode = GD_MOVE_GAUSSIAN;
   for (int i = 0; i < 6; ++i) mc.move_scale[i] = i < 3 ? 0.2f : 0.05f;
   mc.seed = 1;
+  return mc;
+}
+}  // namespace
+
+// One job per launch: atoms per lane {4, 8} x protein tile counts, each run
+// for `steps` MC steps; the fastest mean pair launch (HIP events) wins and is
+// applied to this job (and only to it).
+extern "C" gd_status gd_autotune(gd_job *j, int steps, gd_tiling *best, double *best_ms) {
+  if (!j || steps <= 0) return GD_ERR_ARG;
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  const GdJob g0 = j->h;
+  const int P = g0.P;
+  std::vector<std::pair<int, int>> cand;  // (lc, ptile)
+  for (int lc : {4, 8})
+    for (int nP : {40, 48, 56, 64, 72, 80, 88, 96, 112, 128, 160}) {
+      const int pt = (P + nP - 1) / nP;
+      if (pt >= 4 && std::find(cand.begin(), cand.end(), std::make_pair(lc, pt)) == cand.end())
+        cand.push_back({lc, pt});
+    }
+  {
+    int dlc, dpt;
+    default_tiling(g0.L, P, g0.R, dlc, dpt);
+    if (std::find(cand.begin(), cand.end(), std::make_pair(dlc, dpt)) == cand.end())
+      cand.push_back({dlc, dpt});
+  }
+  const gd_mc_param mc = tune_mc(steps);
   double best_t = 1e300;
   std::pair<int, int> bestc = {g0.lc, g0.ptile};
   for (auto &c : cand) {
@@ -455,16 +442,66 @@ extern "C" gd_status gd_autotune(gd_job *j, int steps, gd_tiling *best, double *
   }
   if ((st = apply_tiling(j, bestc.first, bestc.second)) != GD_OK) return st;
   if ((st = sync_job_struct(j)) != GD_OK) return st;
-  {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[tune_bucket(L, P, g0.R)] = bestc;
-  }
   if (best) {
     best->atoms_per_lane = bestc.first;
     best->tile_points = bestc.second;
-    best->waves_per_cu = pair_waves_per_cu(bestc.first);
+    best->waves_per_cu = 0;
   }
   if (best_ms) *best_ms = best_t;
+  return GD_OK;
+}
+
+// Shared launches: protein tile points x persistent-grid waves per CU (4 atoms
+// per lane), each candidate a whole batched run of `steps` MC steps of the
+// given jobs (graph-captured as gd_batch_dock runs them); the shortest run
+// wins.  The jobs are left at the shared tiling of the table.
+extern "C" gd_status gd_autotune_shared(gd_job **jobs, int n_jobs, int steps, gd_tiling *best,
+                                        double *best_ms) {
+  if (!jobs || n_jobs <= 0 || steps <= 0) return GD_ERR_ARG;
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  std::vector<gd_mc_param> mc(n_jobs, tune_mc(steps));
+  for (int k = 0; k < n_jobs; ++k) mc[k].job_id = (uint32_t)k;
+  double best_t = 1e300;
+  int best_pt = 0, best_w = 0;
+  for (int pt : {32, 40, 48, 64, 80, 96, 128})
+    for (int w : {16, 24, 32}) {
+      if ((st = batch_dock_impl(jobs, n_jobs, mc.data(), nullptr, nullptr, pt, w)) != GD_OK) return st;
+      const auto t0 = std::chrono::steady_clock::now();
+      if ((st = batch_dock_impl(jobs, n_jobs, mc.data(), nullptr, nullptr, pt, w)) != GD_OK) return st;
+      const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      if (t < best_t) {
+        best_t = t;
+        best_pt = pt;
+        best_w = w;
+      }
+    }
+  for (int k = 0; k < n_jobs; ++k)
+    if ((st = shared_tiling(jobs[k], 0)) != GD_OK) return st;
+  if (best) {
+    best->atoms_per_lane = kSharedLc;
+    best->tile_points = best_pt;
+    best->waves_per_cu = best_w;
+  }
+  if (best_ms) *best_ms = best_t;
+  return GD_OK;
+}
+
+// The tiling the library gives a job of these sizes: one job per launch
+// (shared == 0) or inside shared launches.
+extern "C" gd_status gd_tiling_lookup(int L, int P, int R, int shared, gd_tiling *t) {
+  if (!t || L < 1 || P < 1 || R < 1) return GD_ERR_ARG;
+  if (shared) {
+    t->atoms_per_lane = kSharedLc;
+    t->tile_points = shared_ptile(L, P);
+    t->waves_per_cu = pair_waves_per_cu(kSharedLc);
+  } else {
+    int lc, pt;
+    default_tiling(L, P, R, lc, pt);
+    t->atoms_per_lane = lc;
+    t->tile_points = pt;
+    t->waves_per_cu = 0;
+  }
   return GD_OK;
 }
 
@@ -485,6 +522,8 @@ struct ServJob {
   int step0 = 0;
   double t_submit = 0, t_admit = 0;
   bool init_done = false;
+  bool converged = false;
+  std::vector<float> best_hist;  // lowest accepted energy at each sweep's end
   ~ServJob() {
     if (host) gd_host_job_free(host);
   }
@@ -525,19 +564,25 @@ struct HeldOut {
 
 struct gd_server {
   gd_server_config cfg;
+  // mu: pending, results, used, next_ticket, held, cancelled, loading
+  std::mutex mu;
   std::deque<ServJob *> pending;
-  std::vector<ServJob *> active;
   std::vector<gd_job_result> results;
   size_t used = 0;
   int64_t next_ticket = 1;
+  std::map<int64_t, HeldOut *> held;
+  std::set<int64_t> cancelled;   // file requests cancelled while the loader parsed them
+  int64_t loading = 0;           // ticket the loader is parsing (0: none)
+  // smu: serialises gd_server_step with cancellations of active jobs, and
+  // guards active, lists, dirty, launch_step (never held by submit or results)
+  std::mutex smu;
+  std::vector<ServJob *> active;
   int launch_step = 0;
   hipStream_t stream = nullptr;
   BatchLists lists;
   bool dirty = true;
   int lc = kSharedLc;
   std::chrono::steady_clock::time_point t0;
-  std::mutex mu;                 // everything above, and held
-  std::map<int64_t, HeldOut *> held;
   // host input pipeline: file requests parsed by a loader thread
   std::mutex fmu;                // files, stop
   std::condition_variable fcv;
@@ -550,12 +595,12 @@ struct gd_server {
 };
 
 namespace gdr {
-size_t footprint(const gd_job_desc *d);
+size_t footprint(const gd_job_desc *d, bool shared);
 }
 
 // Loader thread: parses file requests outside every lock but its queue's, so
-// parsing overlaps the device work of gd_server_step (which holds mu), then
-// queues the parsed job for admission.
+// parsing overlaps the device work of gd_server_step, then queues the parsed
+// job for admission (or reports it cancelled, if it was while parsing).
 static void server_loader(gd_server *s) {
   for (;;) {
     FileReq *r;
@@ -565,6 +610,8 @@ static void server_loader(gd_server *s) {
       if (s->stop) return;
       r = s->files.front();
       s->files.pop_front();
+      std::lock_guard<std::mutex> lm(s->mu);
+      s->loading = r->ticket;
     }
     gd_host_job *h = nullptr;
     gd_job_desc d;
@@ -574,11 +621,13 @@ static void server_loader(gd_server *s) {
                                    r->ene, r->n_tmp, r->floor_temp, r->ceiling_temp, &h);
     if (st == GD_OK) st = gd_host_job_desc(h, &d);
     if (st == GD_OK) {
-      foot = footprint(&d);
+      foot = footprint(&d, true);
       if (foot == 0) st = GD_ERR_ARG;
       else if (foot > s->cfg.hbm_budget) st = GD_ERR_BUSY;  // can never be admitted
     }
     std::lock_guard<std::mutex> lk(s->mu);
+    s->loading = 0;
+    if (s->cancelled.erase(r->ticket)) st = GD_ERR_STATE;  // cancelled while parsed
     if (st != GD_OK) {
       if (h) gd_host_job_free(h);
       s->results.push_back({r->ticket, st, r->t_submit, 0.0, s->now(), 0.0});
@@ -602,7 +651,9 @@ static void server_loader(gd_server *s) {
 
 
 extern "C" gd_status gd_server_create(const gd_server_config *cfg, gd_server **out) {
-  if (!cfg || !out || cfg->sweep_steps <= 0 || cfg->max_active <= 0) return GD_ERR_ARG;
+  if (!cfg || !out || cfg->sweep_steps <= 0 || cfg->max_active <= 0 || cfg->converge_sweeps < 0 ||
+      !(cfg->converge_tol >= 0.0f))
+    return GD_ERR_ARG;
   gd_status st = ensure_device();
   if (st != GD_OK) return st;
   gd_server *s = new gd_server();
@@ -643,16 +694,16 @@ extern "C" double gd_server_clock(gd_server *s) { return s ? s->now() : 0.0; }
 extern "C" gd_status gd_server_submit(gd_server *s, const gd_job_desc *desc, const gd_mc_param *mc,
                                       gd_replica_out *out, int64_t *ticket) {
   if (!s || !desc || !mc || mc->steps_total < 0) return GD_ERR_ARG;
-  const size_t foot = footprint(desc);
+  const size_t foot = footprint(desc, true);
   if (foot == 0) return GD_ERR_ARG;
   if (foot > s->cfg.hbm_budget) return GD_ERR_BUSY;  // can never be admitted
-  std::lock_guard<std::mutex> lk(s->mu);
   ServJob *j = new ServJob();
-  j->ticket = s->next_ticket++;
   j->desc = *desc;
   j->mc = *mc;
   j->out = out;
   j->foot = foot;
+  std::lock_guard<std::mutex> lk(s->mu);  // never held across a device sweep
+  j->ticket = s->next_ticket++;
   j->t_submit = s->now();
   s->pending.push_back(j);
   if (ticket) *ticket = j->ticket;
@@ -717,52 +768,63 @@ extern "C" gd_status gd_server_take(gd_server *s, int64_t ticket, int *n_rep, gd
   return GD_OK;
 }
 
+namespace {
+// A request that will not run: its result is the status, and a file request
+// leaves nothing to take.  Caller holds mu.
+void fail_request(gd_server *s, int64_t ticket, gd_status st, double t_submit, double t_admit) {
+  s->results.push_back({ticket, st, t_submit, t_admit, s->now(), 0.0});
+  auto h = s->held.find(ticket);
+  if (h != s->held.end()) {
+    delete h->second;
+    s->held.erase(h);
+  }
+}
+}  // namespace
+
 extern "C" gd_status gd_server_cancel(gd_server *s, int64_t ticket) {
   if (!s) return GD_ERR_ARG;
-  {  // a file request not parsed yet
-    FileReq *r = nullptr;
-    {
-      std::lock_guard<std::mutex> lk(s->fmu);
-      for (auto it = s->files.begin(); it != s->files.end(); ++it)
-        if ((*it)->ticket == ticket) {
-          r = *it;
-          s->files.erase(it);
-          break;
-        }
-    }
-    if (r) {
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->results.push_back({ticket, GD_ERR_STATE, r->t_submit, 0.0, s->now(), 0.0});
-      delete r;
+  {  // a file request not parsed yet, or being parsed
+    std::lock_guard<std::mutex> lf(s->fmu);
+    for (auto it = s->files.begin(); it != s->files.end(); ++it)
+      if ((*it)->ticket == ticket) {
+        FileReq *r = *it;
+        s->files.erase(it);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->results.push_back({ticket, GD_ERR_STATE, r->t_submit, 0.0, s->now(), 0.0});
+        delete r;
+        return GD_OK;
+      }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->loading == ticket) {  // the loader reports it cancelled when done parsing
+      s->cancelled.insert(ticket);
       return GD_OK;
     }
   }
-  std::lock_guard<std::mutex> lk(s->mu);
-  {
-    auto h = s->held.find(ticket);
-    if (h != s->held.end() && !h->second->done) {  // cancelled: nothing to take
-      delete h->second;
-      s->held.erase(h);
-    }
+  {  // a pending request
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto it = s->pending.begin(); it != s->pending.end(); ++it)
+      if ((*it)->ticket == ticket) {
+        ServJob *j = *it;
+        s->pending.erase(it);
+        fail_request(s, ticket, GD_ERR_STATE, j->t_submit, 0.0);
+        delete j;
+        return GD_OK;
+      }
   }
-  for (auto it = s->pending.begin(); it != s->pending.end(); ++it)
-    if ((*it)->ticket == ticket) {
-      s->results.push_back({ticket, GD_ERR_STATE, (*it)->t_submit, 0.0, s->now(), 0.0});
-      delete *it;
-      s->pending.erase(it);
-      return GD_OK;
-    }
+  // an active job: wait for a running sweep
+  std::lock_guard<std::mutex> ls(s->smu);
   for (auto it = s->active.begin(); it != s->active.end(); ++it)
     if ((*it)->ticket == ticket) {
       (void)hipStreamSynchronize(s->stream);
       ServJob *j = *it;
-      s->results.push_back({ticket, GD_ERR_STATE, j->t_submit, j->t_admit, s->now(),
-                            (double)j->job->h.R * j->done_steps});
+      s->active.erase(it);
+      s->dirty = true;
+      std::lock_guard<std::mutex> lk(s->mu);
+      fail_request(s, ticket, GD_ERR_STATE, j->t_submit, j->t_admit);
+      s->results.back().replica_steps = (double)j->job->h.R * j->done_steps;
       s->used -= j->foot;
       gd_job_destroy(j->job);
       delete j;
-      s->active.erase(it);
-      s->dirty = true;
       return GD_OK;
     }
   return GD_ERR_ARG;
@@ -777,7 +839,7 @@ extern "C" gd_status gd_server_usage(gd_server *s, size_t *used, int *n_pending)
   }
   std::lock_guard<std::mutex> lk(s->mu);
   if (used) *used = s->used;
-  if (n_pending) *n_pending = (int)s->pending.size() + nf;
+  if (n_pending) *n_pending = (int)s->pending.size() + nf + (s->loading ? 1 : 0);
   return GD_OK;
 }
 
@@ -791,53 +853,61 @@ extern "C" gd_status gd_server_results(gd_server *s, gd_job_result *res, int cap
   return GD_OK;
 }
 
+namespace {
+// Lowest energy any replica of the job has accepted (device best[] array).
+gd_status job_best(gd_job *j, hipStream_t s, float *out) {
+  std::vector<float> b(j->h.R);
+  hipError_t e = hipMemcpyAsync(b.data(), j->h.best, b.size() * sizeof(float), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy best");
+  float m = b.empty() ? 0.0f : b[0];
+  for (float v : b) m = std::min(m, v);
+  *out = m;
+  return GD_OK;
+}
+}  // namespace
+
 extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
   if (!s) return GD_ERR_ARG;
-  std::lock_guard<std::mutex> lk(s->mu);
+  std::lock_guard<std::mutex> ls(s->smu);
   gd_status st = ensure_device();
   if (st != GD_OK) return st;
   hipStream_t hs = s->stream;
-  // (1) admission: FIFO while the footprint fits and a slot is free
+  // (1) admission: FIFO while the estimated footprint fits and a slot is free;
+  // an admitted job is then charged its actual device bytes
   std::vector<ServJob *> admitted;
-  while (!s->pending.empty() && (int)s->active.size() < s->cfg.max_active &&
-         s->used + s->pending.front()->foot <= s->cfg.hbm_budget) {
-    ServJob *j = s->pending.front();
-    s->pending.pop_front();
-    auto drop_held = [&](int64_t t) {  // a failed file request leaves nothing to take
-      auto h = s->held.find(t);
-      if (h != s->held.end()) {
-        delete h->second;
-        s->held.erase(h);
-      }
-    };
+  for (;;) {
+    ServJob *j = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->pending.empty() || (int)s->active.size() + (int)admitted.size() >= s->cfg.max_active ||
+          s->used + s->pending.front()->foot > s->cfg.hbm_budget)
+        break;
+      j = s->pending.front();
+      s->pending.pop_front();
+      s->used += j->foot;  // held while the job is created
+    }
     st = gd_job_create(&j->desc, &j->job);
+    if (st == GD_OK) st = shared_tiling(j->job, 0);  // serving tiling: a function of the complex only
+    if (st == GD_OK) {
+      set_mc(j->job->h, &j->mc);
+      j->job->h.record = 0;
+      j->job->h.step0 = s->launch_step;
+      st = reset_replicas(j->job);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->used -= j->foot;
     if (st != GD_OK) {
-      s->results.push_back({j->ticket, st, j->t_submit, s->now(), s->now(), 0.0});
-      drop_held(j->ticket);
+      fail_request(s, j->ticket, st, j->t_submit, s->now());
+      if (j->job) gd_job_destroy(j->job);
       delete j;
       continue;
     }
-    // serving tiling: a function of the complex only
-    {
-      if ((st = apply_tiling(j->job, s->lc, shared_ptile(j->job->h.P))) != GD_OK ||
-          (st = sync_job_struct(j->job)) != GD_OK) {
-        s->results.push_back({j->ticket, st, j->t_submit, s->now(), s->now(), 0.0});
-        drop_held(j->ticket);
-        gd_job_destroy(j->job);
-        delete j;
-        continue;
-      }
-    }
-    set_mc(j->job->h, &j->mc);
-    j->job->h.record = 0;
-    if ((st = reset_replicas(j->job)) != GD_OK) return st;
+    j->foot = gd_job_device_bytes(j->job);
+    s->used += j->foot;
     j->t_admit = s->now();
     j->step0 = s->launch_step;
-    j->job->h.step0 = s->launch_step;
-    s->used += j->foot;
-    s->active.push_back(j);
     admitted.push_back(j);
-    s->dirty = true;
   }
   // (2) initial-energy pass of the newly admitted jobs, in one batch
   if (!admitted.empty()) {
@@ -848,18 +918,34 @@ extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
       nj.push_back(j->job);
     }
     BatchLists B;
-    if ((st = B.build(nj, idx, hs)) != GD_OK) return st;
-    if ((st = batch_init(B, s->lc, pair_grid(pair_waves_per_cu(s->lc)), hs)) != GD_OK) return st;
-    hipError_t e = hipStreamSynchronize(hs);
-    if (e != hipSuccess) return hip_fail(e, "init pass");
+    st = B.build(nj, idx, hs);
+    if (st == GD_OK) st = batch_init(B, s->lc, pair_grid(pair_waves_per_cu(s->lc)), hs);
+    if (st == GD_OK) {
+      hipError_t e = hipStreamSynchronize(hs);
+      if (e != hipSuccess) st = hip_fail(e, "init pass");
+    }
+    if (st != GD_OK) {  // none of them runs: report each
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (ServJob *j : admitted) {
+        fail_request(s, j->ticket, st, j->t_submit, j->t_admit);
+        s->used -= j->foot;
+        gd_job_destroy(j->job);
+        delete j;
+      }
+      return st;
+    }
+    for (ServJob *j : admitted) s->active.push_back(j);
+    s->dirty = true;
   }
-  // (3) advance every active job by up to sweep_steps
+  // (3) advance every active job by up to sweep_steps (no lock held but smu:
+  // submits and result queries proceed meanwhile)
   for (int k = 0; k < s->cfg.sweep_steps && st == GD_OK; ++k) {
     std::vector<int> act;
     std::vector<gd_job *> jobs;
     for (ServJob *j : s->active) jobs.push_back(j->job);
     for (int q = 0; q < (int)s->active.size(); ++q)
-      if (s->active[q]->done_steps < s->active[q]->mc.steps_total) act.push_back(q);
+      if (s->active[q]->done_steps < s->active[q]->mc.steps_total && !s->active[q]->converged)
+        act.push_back(q);
     if (act.empty()) break;
     if (s->dirty || (int)act.size() != s->lists.n_jobs) {
       for (int q : act) jobs[q]->h.step0 = s->active[q]->step0;
@@ -887,25 +973,44 @@ extern "C" gd_status gd_server_step(gd_server *s, int *n_active, int *n_done) {
   hipError_t e = hipStreamSynchronize(hs);
   if (st == GD_OK && e != hipSuccess) st = hip_fail(e, "sweep");
   if (st != GD_OK) return st;
-  // (4) retire finished jobs
+  // (4) convergence: a job's sweep ends every sweep_steps of its own steps
+  if (s->cfg.converge_sweeps > 0) {
+    for (ServJob *j : s->active) {
+      if (j->converged || j->done_steps >= j->mc.steps_total || j->done_steps == 0 ||
+          j->done_steps % s->cfg.sweep_steps != 0)
+        continue;
+      float b;
+      if ((st = job_best(j->job, hs, &b)) != GD_OK) return st;
+      j->best_hist.push_back(b);
+      const int n = (int)j->best_hist.size(), w = s->cfg.converge_sweeps;
+      if (n > w && j->best_hist[n - 1 - w] - b < s->cfg.converge_tol) j->converged = true;
+    }
+  }
+  // (5) retire finished and converged jobs
+  std::vector<ServJob *> done;
   for (auto it = s->active.begin(); it != s->active.end();) {
     ServJob *j = *it;
-    if (j->done_steps >= j->mc.steps_total) {
-      gd_status rs = read_out(j->job, j->out);
-      auto h = s->held.find(j->ticket);
-      if (h != s->held.end()) h->second->done = true;
-      s->results.push_back({j->ticket, rs, j->t_submit, j->t_admit, s->now(),
-                            (double)j->job->h.R * j->mc.steps_total});
-      s->used -= j->foot;
-      gd_job_destroy(j->job);
-      delete j;
+    if (j->done_steps >= j->mc.steps_total || j->converged) {
+      done.push_back(j);
       it = s->active.erase(it);
       s->dirty = true;
     } else {
       ++it;
     }
   }
+  for (ServJob *j : done) {
+    const gd_status rs = read_out(j->job, j->out);
+    const double rsteps = (double)j->job->h.R * j->done_steps;
+    gd_job_destroy(j->job);
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto h = s->held.find(j->ticket);
+    if (h != s->held.end()) h->second->done = true;
+    s->results.push_back({j->ticket, rs, j->t_submit, j->t_admit, s->now(), rsteps});
+    s->used -= j->foot;
+    delete j;
+  }
   if (n_active) *n_active = (int)s->active.size();
+  std::lock_guard<std::mutex> lk(s->mu);
   if (n_done) *n_done = (int)s->results.size();
   return GD_OK;
 }

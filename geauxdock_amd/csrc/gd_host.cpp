@@ -18,6 +18,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <random>
 #include <sstream>
@@ -187,11 +188,14 @@ extern "C" gd_status gd_load_enepara(const char *paras, const char *weight, cons
   if (!paras || !weight || !nor_a || !nor_b || !out) return GD_ERR_ARG;
   std::vector<std::string> lines;
   if (!read_lines(paras, lines)) return GD_ERR_IO;
-  // raw tables (EnePara0 of geauxdock.h:279-295), zero-initialised
-  static float vdw[GD_NPT][GD_NLT][2], pmf[GD_NPT][GD_NLT][2], hdb[GD_NPT][GD_NLT][2];
-  std::memset(vdw, 0, sizeof vdw);
-  std::memset(pmf, 0, sizeof pmf);
-  std::memset(hdb, 0, sizeof hdb);
+  // raw tables (EnePara0 of geauxdock.h:279-295), zero-initialised, owned by
+  // this call (concurrent loads share nothing)
+  struct RawTables {
+    float vdw[GD_NPT][GD_NLT][2], pmf[GD_NPT][GD_NLT][2], hdb[GD_NPT][GD_NLT][2];
+  };
+  std::unique_ptr<RawTables> raw(new RawTables());
+  std::memset(raw.get(), 0, sizeof(RawTables));
+  float(*vdw)[GD_NLT][2] = raw->vdw, (*pmf)[GD_NLT][2] = raw->pmf, (*hdb)[GD_NLT][2] = raw->hdb;
   float ele[GD_NELE] = {0}, hpp[GD_NRES] = {0}, hpl[GD_NLT][2] = {{0}};
   float lj[3] = {0, 0, 0}, el[2] = {0, 0}, kde = 0;
   float w[GD_NTERM] = {0}, a_para[GD_NTERM] = {0}, b_para[GD_NTERM] = {0};
@@ -777,12 +781,17 @@ V3 rot_about(V3 p, V3 o, V3 k, double a) {
 }
 }  // namespace
 
-extern "C" gd_status gd_synth_complex(int P, int L, int n_lig, int n_prt, int n_tmp,
-                                      float floor_temp, float ceiling_temp, uint64_t seed,
-                                      const gd_enepara *ene, gd_host_job **out) {
+// The synthetic complex of the benchmark workloads.  rl draws the ligand and
+// its restraints (KDE, MCS, PSP), rp the protein and its charges; with one
+// generator for both the draws interleave in that order.  pocket_r > 0 fixes
+// the radius of the protein's empty pocket (one protein for many ligands),
+// else it is the ligand's radius + 2.5 A.
+static gd_status synth(int P, int L, int n_lig, int n_prt, int n_tmp, float floor_temp,
+                       float ceiling_temp, Rng &rl, Rng &rp, double pocket_r_fixed,
+                       const gd_enepara *ene, gd_host_job **out) {
   if (!ene || !out || P < 8 || L < 2 || n_lig < 1 || n_prt < 1 || n_tmp < 1) return GD_ERR_ARG;
   *out = nullptr;
-  Rng rng(seed * 0x9E3779B97F4A7C15ull + 0x1234567ull);
+  Rng &rng = rl;
   gd_host_job *h = new gd_host_job();
   h->ene = *ene;
   const double box = 40.0 * std::cbrt(P / 1200.0);
@@ -875,7 +884,10 @@ extern "C" gd_status gd_synth_complex(int P, int L, int n_lig, int n_prt, int n_
 
   // ---- protein: residue chains by random walk, expanded into points
   Prt0 pr;
-  const double pocket_r = lig_radius + 2.5;
+  const double pocket_r = pocket_r_fixed > 0.0 ? pocket_r_fixed : lig_radius + 2.5;
+  std::vector<V3> cas;  // CA of each residue
+  {
+  Rng &rng = rp;
   auto inside = [&](V3 p) {
     return p.x > 0 && p.y > 0 && p.z > 0 && p.x < box && p.y < box && p.z < box &&
            norm(p - center) > pocket_r;
@@ -883,7 +895,6 @@ extern "C" gd_status gd_synth_complex(int P, int L, int n_lig, int n_prt, int n_
   V3 ca{0, 0, 0};
   bool new_chain = true;
   int serial = 0;
-  std::vector<V3> cas;
   std::vector<V3> pts;
   while ((int)pr.t.size() < P) {
     if (new_chain) {
@@ -945,6 +956,7 @@ extern "C" gd_status gd_synth_complex(int P, int L, int n_lig, int n_prt, int n_
     }
     for (int q = 0; q < P; ++q) h->prt_ele[q] = (float)(h->prt_ele[q] - s / P);
   }
+  }  // protein draws
 
   // ---- KDE: 120 per atom
   std::vector<int> kt;
@@ -997,6 +1009,25 @@ extern "C" gd_status gd_synth_complex(int P, int L, int n_lig, int n_prt, int n_
   h->raw_conf = n_lig - 1;
   *out = h;
   return GD_OK;
+}
+
+extern "C" gd_status gd_synth_complex(int P, int L, int n_lig, int n_prt, int n_tmp,
+                                      float floor_temp, float ceiling_temp, uint64_t seed,
+                                      const gd_enepara *ene, gd_host_job **out) {
+  Rng rng(seed * 0x9E3779B97F4A7C15ull + 0x1234567ull);
+  return synth(P, L, n_lig, n_prt, n_tmp, floor_temp, ceiling_temp, rng, rng, 0.0, ene, out);
+}
+
+// A complex of a virtual screen: the protein of protein_seed (the same target
+// for every ligand_seed, its pocket a fixed 14 A cavity: the generator's
+// 80-atom ligands reach about 11.5 A from their centroid) with the ligand of
+// ligand_seed.
+extern "C" gd_status gd_synth_screen(int P, uint64_t protein_seed, int L, uint64_t ligand_seed,
+                                     int n_lig, int n_prt, int n_tmp, float floor_temp,
+                                     float ceiling_temp, const gd_enepara *ene, gd_host_job **out) {
+  Rng rl(ligand_seed * 0x9E3779B97F4A7C15ull + 0x1234567ull);
+  Rng rp(protein_seed * 0xD1B54A32D192ED03ull + 0x7F4A7C15ull);
+  return synth(P, L, n_lig, n_prt, n_tmp, floor_temp, ceiling_temp, rl, rp, 14.0, ene, out);
 }
 
 extern "C" gd_status gd_host_job_desc(const gd_host_job *h, gd_job_desc *d) {

@@ -16,7 +16,9 @@ struct gd_job {
   GdJob *d_job = nullptr;    // device copy
   void *d_base = nullptr;    // tables + replica state
   void *d_part = nullptr;    // partial sums (tiling-dependent)
+  size_t part_bytes = 0;
   void *d_lists = nullptr;   // work items + blocks
+  size_t lists_bytes = 0;
   void *d_rec = nullptr;     // accept records: a two-half ring, one half per dump
   size_t rec_bytes = 0;
   void *h_rec = nullptr;     // pinned host copies of both halves (record streaming)
@@ -43,8 +45,12 @@ namespace gdr {
 void set_error(const std::string &s);
 gd_status hip_fail(hipError_t e, const char *what);
 gd_status ensure_device();
+int device();   // the device the library runs on (-1 before initialisation)
 int n_cu();
-void default_tiling(int L, int P, int R, int &lc, int &ptile);
+void default_tiling(int L, int P, int R, int &lc, int &ptile);  // one job per launch
+int shared_ptile(int L, int P);      // tile points in shared launches (4 atoms per lane)
+int pair_waves_per_cu(int lc);       // persistent pair kernel residency
+int lg2_floor(int x);
 gd_status apply_tiling(gd_job *j, int lc, int ptile);
 gd_status sync_job_struct(gd_job *j);
 gd_status reset_replicas(gd_job *j);

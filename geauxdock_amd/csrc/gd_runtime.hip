@@ -25,6 +25,7 @@
 #include "../../include/geauxdock_amd.h"
 #include "gd_device.h"
 #include "gd_internal.h"
+#include "gd_tiling_table.h"
 
 // kernels (gd_kernels.hip)
 template <int LC>
@@ -184,13 +185,42 @@ static float near_r2_threshold(double pmf0, double el1, float psp_thr, bool hb, 
 }
 
 // ---------------------------------------------------------------------------
-// Tiling: ligand chunk size lc (kernel template), protein tile ptile.  The
-// default aims for ~22 waves per SIMD over the whole launch; it depends only
-// on the job so a job's reduction order is the same alone or in a batch.
-bool tuned_tiling(int L, int P, int R, int &lc, int &ptile);  // gd_batch.hip
+// Tiling: ligand chunk size lc (kernel template), protein tile ptile.  Looked
+// up by size bucket in the tuned table (gd_tiling_table.h), else the rule
+// below; either way a function of the job (and the device's CU count) only, so
+// a job's reduction order, and hence its bits, is the same in every process,
+// alone or in a batch.
+int lg2_floor(int x) {
+  int b = 0;
+  while (b < 30 && (2 << b) <= x) ++b;
+  return b;
+}
+
+static const GdTuned *tuned_entry(int mode, int L, int P, int R) {
+  const int lgL = lg2_floor(std::max(1, L)), lgP = lg2_floor(std::max(1, P)),
+            lgR = lg2_floor(std::max(1, R));
+  for (const GdTuned &t : kTunedTable) {
+    if (t.mode != mode) continue;
+    if (mode == GD_TUNE_WAVES ||
+        (t.lgL == lgL && t.lgP == lgP && (mode == GD_TUNE_SHARED || t.lgR == lgR)))
+      return &t;
+  }
+  return nullptr;
+}
+
+// ptile giving at least nP tiles of P points (the kernel rounds nP up to a
+// multiple of GD_GROUP)
+static int ptile_for_tiles(int P, int nP) {
+  nP = std::max(1, std::min(nP, P));
+  return (P + nP - 1) / nP;
+}
 
 void default_tiling(int L, int P, int R, int &lc, int &ptile) {
-  if (tuned_tiling(L, P, R, lc, ptile)) return;
+  if (const GdTuned *t = tuned_entry(GD_TUNE_PLAIN, L, P, R)) {
+    lc = t->lc;
+    ptile = ptile_for_tiles(P, t->value);
+    return;
+  }
   // 4 atoms per wave keeps the pair kernel at ~62 VGPRs (8 waves per SIMD);
   // ~22 waves per SIMD over the launch (about three waves per slot) balanced
   // per-tile overhead (KDE and MCS shares, partial sums) against the launch's
@@ -206,6 +236,23 @@ void default_tiling(int L, int P, int R, int &lc, int &ptile) {
   ptile = (P + nP - 1) / nP;
 }
 
+// Tiling of a job inside shared launches (batches, serving): 4 atoms per lane
+// (one kernel template for every job of a launch) and the bucket's tile size,
+// 64 points where the table has none.
+int shared_ptile(int L, int P) {
+  if (const GdTuned *t = tuned_entry(GD_TUNE_SHARED, L, P, 0)) return std::min(P, t->value);
+  return std::min(P, 64);
+}
+
+// Resident waves per CU of the persistent pair kernel for lc atoms per lane:
+// the tuned value for 4 atoms, else as many 8-wave groups as the registers
+// allow (4 atoms: ~60 VGPRs, 8 waves per SIMD).
+int pair_waves_per_cu(int lc) {
+  if (lc == 4)
+    if (const GdTuned *t = tuned_entry(GD_TUNE_WAVES, 0, 0, 0)) return t->value;
+  return lc == 4 ? 32 : lc == 8 ? 20 : 8;
+}
+
 static gd_status alloc_partials(gd_job *j, GdJob &g, int cap_rp) {
   // partials depend on the tiling; they live in their own allocation
   if (j->d_part) {
@@ -218,7 +265,9 @@ static gd_status alloc_partials(gd_job *j, GdJob &g, int cap_rp) {
   a.add(&g.part_hpc, (size_t)g.L * g.nG * cap_rp);
   a.add(&g.part_mcs, (size_t)std::max(1, g.M) * g.nL * cap_rp);
   a.add(&g.fin_part, (size_t)(cap_rp / 64) * GD_RK * 8 * 64);
-  return a.commit(&j->d_part);
+  gd_status st = a.commit(&j->d_part);
+  j->part_bytes = st == GD_OK ? a.off : 0;
+  return st;
 }
 
 // Device order of the protein points for the job's tile count: class by
@@ -584,7 +633,9 @@ static gd_status pack_job(const gd_job_desc *d, gd_job *j) {
 // zero pose and energies; padding lanes get conformer 0 and never write back.
 // Device bytes gd_job_create allocates for a descriptor (tables, replica
 // state, partial sums under the default tiling, work lists).
-size_t footprint(const gd_job_desc *d) {
+// Device bytes of a complex under the plain tiling, or (shared) under the
+// tiling of shared launches.
+size_t footprint(const gd_job_desc *d, bool shared) {
   if (!d || d->lig_natom < 1 || d->prt_npoint < 1 || d->n_lig < 1 || d->n_prt < 1 || d->n_tmp < 1)
     return 0;
   const size_t L = d->lig_natom, P = d->prt_npoint, K = d->kde_npoint, M = d->mcs_nrow;
@@ -602,9 +653,14 @@ size_t footprint(const gd_job_desc *d) {
              a(std::max<size_t>(1, M) * 8) + a(nt * 4);
   b += a(6 * Rp * 4) * 2 + a(10 * Rp * 4) + a(Rp * 4) * 6 + a(2 * Rp * 4) + a(Rp * 4) +
        a(Rp * 4) + a(nl * np * nt * 4) +
-       a(3 * L * Rp * 4) + a(Rp * 4) + a(10 * Rp * 4) + a(9 * Rp * 4) + a(64 * 4);
+       a(3 * L * Rp * 4) + a(2 * Rp * 4) + a(10 * Rp * 4) + a(9 * Rp * 4) + a(64 * 4);
   int lc, ptile;
-  default_tiling((int)L, (int)P, (int)R, lc, ptile);
+  if (shared) {
+    lc = 4;
+    ptile = shared_ptile((int)L, (int)P);
+  } else {
+    default_tiling((int)L, (int)P, (int)R, lc, ptile);
+  }
   const size_t nL = (L + lc - 1) / lc, nP0 = (P + ptile - 1) / ptile;
   const size_t nG = (nP0 + GD_GROUP - 1) / GD_GROUP;
   b += a(GD_NLIN * nL * nG * Rp * 4) + a(L * nG * Rp * 4) + a(std::max<size_t>(1, M) * nL * Rp * 4) +
@@ -660,8 +716,10 @@ gd_status sync_job_struct(gd_job *j) {
   if (j->d_lists) hipFree(j->d_lists);
   j->d_lists = nullptr;
   const size_t bi = (size_t)j->n_items * sizeof(GdItem), bb = (size_t)j->n_blocks * sizeof(GdBlock);
+  j->lists_bytes = 0;
   e = hipMalloc(&j->d_lists, bi + bb + 256);
   if (e != hipSuccess) return hip_fail(e, "hipMalloc lists");
+  j->lists_bytes = bi + bb + 256;
   j->d_items = (GdItem *)j->d_lists;
   j->d_blocks = (GdBlock *)((char *)j->d_lists + (bi + 255) / 256 * 256);
   if ((e = hipMemcpyAsync(j->d_items, j->items.data(), bi, hipMemcpyHostToDevice, j->stream)) != hipSuccess)
@@ -911,15 +969,13 @@ gd_status gd_job_sizes(const gd_job *j, int *n_rep, int *L, int *P, int *K, int 
   return GD_OK;
 }
 
-size_t gd_job_footprint(const gd_job_desc *d) { return footprint(d); }
+size_t gd_job_footprint(const gd_job_desc *d) { return footprint(d, false); }
 
+// Device memory the job holds: tables and replica state, partial sums, work
+// lists, the record ring (when a run recorded) and the device job struct.
 size_t gd_job_device_bytes(const gd_job *j) {
   if (!j) return 0;
-  const GdJob &g = j->h;
-  const size_t part = ((size_t)GD_NLIN * g.nL * g.nG + (size_t)g.L * g.nG + (size_t)g.M * g.nL +
-                       (size_t)GD_RK * 8) *
-                      g.Rp * sizeof(float);
-  return j->bytes + part;
+  return j->bytes + j->part_bytes + j->lists_bytes + j->rec_bytes + sizeof(GdJob);
 }
 
 gd_status gd_tiling_get(const gd_job *j, gd_tiling *t) {

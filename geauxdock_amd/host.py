@@ -142,6 +142,17 @@ def synth_complex(prt_npoint, lig_natom, n_lig, n_prt, n_tmp, floor_temp=0.004,
     return HostJob(h, ene)
 
 
+def synth_screen(prt_npoint, protein_seed, lig_natom, ligand_seed, n_lig, n_prt, n_tmp,
+                 floor_temp=0.004, ceiling_temp=0.04, ene=None):
+    """One complex of a virtual screen: the protein target of protein_seed
+    with the ligand of ligand_seed (gd_synth_screen)."""
+    ene = ene or load_enepara()
+    h = C.c_void_p()
+    check(lib().gd_synth_screen(prt_npoint, protein_seed, lig_natom, ligand_seed, n_lig, n_prt, n_tmp,
+                                floor_temp, ceiling_temp, C.byref(ene), C.byref(h)), "gd_synth_screen")
+    return HostJob(h, ene)
+
+
 def make_mc(steps_total, steps_per_dump=None, steps_per_exchange=0, move_mode="gaussian",
             move_scale=None, seed=1, job_id=0, record=False, axes=("temp",)):
     """MC parameters; `axes` names the exchange axes ("temp", "lig", "prt")."""
@@ -159,6 +170,15 @@ def make_mc(steps_total, steps_per_dump=None, steps_per_exchange=0, move_mode="g
     bits = {"temp": abi.GD_EXCH_TEMP, "lig": abi.GD_EXCH_LIG, "prt": abi.GD_EXCH_PRT}
     mc.exchange_axes = sum(bits[a] for a in set(axes))
     return mc
+
+
+def tiling_lookup(lig_natom, prt_npoint, n_rep, shared):
+    """(atoms per lane, tile points, waves per CU) the library gives a job of
+    these sizes, one job per launch or in shared launches (gd_tiling_lookup)."""
+    t = abi.gd_tiling()
+    check(lib().gd_tiling_lookup(lig_natom, prt_npoint, n_rep, 1 if shared else 0, C.byref(t)),
+          "gd_tiling_lookup")
+    return t.atoms_per_lane, t.tile_points, t.waves_per_cu
 
 
 class ReplicaOut:

@@ -248,15 +248,33 @@ typedef struct {
 } gd_tiling;
 gd_status gd_tiling_get(const gd_job *job, gd_tiling *t);
 gd_status gd_tiling_set(gd_job *job, const gd_tiling *t);
-/* Sweep the tiling grid on `job` for `steps` MC steps each and keep the
- * fastest; results cached by (protein-size, ligand-size) bucket. */
+/* Sweep atoms per lane x protein tile counts on `job` for `steps` MC steps
+ * each and apply the fastest to this job only (the reference's counterpart,
+ * the disabled CalcThreadBlockShape of backend_gpu/kernel_cuda_l2_util.cu:40-67,
+ * is a formula). */
 gd_status gd_autotune(gd_job *job, int steps, gd_tiling *best, double *best_ms);
+/* Sweep the tiling of shared launches (protein tile points x persistent-grid
+ * waves per CU, 4 atoms per lane) on a batch of these jobs, `steps` MC steps
+ * per candidate; returns the fastest.  Nothing process-wide changes: the
+ * library's tilings come from its tuned table (geauxdock_amd/tune.py writes
+ * it from these sweeps). */
+gd_status gd_autotune_shared(gd_job **jobs, int n_jobs, int steps, gd_tiling *best, double *best_ms);
+/* The tiling the library gives a job of these sizes, one job per launch
+ * (shared == 0) or in shared launches (batches, serving). */
+gd_status gd_tiling_lookup(int lig_natom, int prt_npoint, int n_rep, int shared, gd_tiling *t);
 
 /* ---- serving ------------------------------------------------------------ */
 typedef struct {
   size_t hbm_budget;        /* bytes of device memory jobs may occupy */
   int max_active;           /* cap on concurrently active jobs */
   int sweep_steps;          /* MC steps per sweep between admissions */
+  /* Convergence retirement (0 = off): a job retires before its step count
+   * once the lowest energy any of its replicas has accepted improved by less
+   * than converge_tol over its last converge_sweeps sweeps.  A job's sweeps
+   * end every sweep_steps of its own steps, so the decision depends on the
+   * job alone, not on what else is active. */
+  int converge_sweeps;
+  float converge_tol;
 } gd_server_config;
 
 typedef struct {
@@ -266,10 +284,16 @@ typedef struct {
   double replica_steps;
 } gd_job_result;
 
+/* A job's result: status GD_OK; replica_steps counts the steps it ran (fewer
+ * than requested when it retired on convergence).  gd_server_step and
+ * gd_server_cancel of active jobs are serialised with each other; submits,
+ * results and usage queries never wait for a device sweep. */
 gd_status gd_server_create(const gd_server_config *cfg, gd_server **out);
 gd_status gd_server_destroy(gd_server *s);
-/* Device bytes a complex will occupy once admitted (host-side estimate that
- * matches what gd_job_create allocates with the default tiling). */
+/* Device bytes a complex will occupy as a plain job (host-side estimate of
+ * what gd_job_create allocates with the default tiling).  The server admits
+ * by the same estimate for its shared-launch tiling and then charges the
+ * job's actual gd_job_device_bytes. */
 size_t gd_job_footprint(const gd_job_desc *desc);
 /* Queue a docking request.  The descriptor's arrays must stay valid until the
  * request is admitted (its results are ready or it was cancelled); out
@@ -330,6 +354,12 @@ gd_status gd_load_complex(const char *sdf, const char *ff, const char *pdb,
 gd_status gd_synth_complex(int prt_npoint, int lig_natom, int n_lig, int n_prt, int n_tmp,
                            float floor_temp, float ceiling_temp, uint64_t seed,
                            const gd_enepara *ene, gd_host_job **out);
+/* One complex of a virtual screen: the protein target of protein_seed (the
+ * same points and pocket for every ligand) with the ligand, KDE anchors, MCS
+ * rows and psp entries of ligand_seed. */
+gd_status gd_synth_screen(int prt_npoint, uint64_t protein_seed, int lig_natom, uint64_t ligand_seed,
+                          int n_lig, int n_prt, int n_tmp, float floor_temp, float ceiling_temp,
+                          const gd_enepara *ene, gd_host_job **out);
 /* View of a host job as a gd_job_desc (pointers stay valid until free). */
 gd_status gd_host_job_desc(const gd_host_job *h, gd_job_desc *desc);
 gd_status gd_host_job_free(gd_host_job *h);

@@ -551,9 +551,11 @@ int or_energy(const gd_job_desc *d, int n, const int *conf, const float *pose,
 // at the start of step s, the replica's accepted (step s, movematrix,
 // energy) when the step before it, the initial pass or a conformer swap was
 // accepted.
+// literal != 0 evaluates energies in the reference's own FP32 expression order
+// (bit-exact against the reference backend) instead of the GPU rounding.
 int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_lo, int rep_hi,
             gd_replica_out *out, float *trace, double *ms, float *rec, int rec_cap,
-            int *rec_count) {
+            int *rec_count, int literal) {
   if (!d || !mc) return GD_ERR_ARG;
   const Job J = make_job(d);
   const int L = J.L;
@@ -608,7 +610,7 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
           u = u01(dr.w[6]);
         }
         transform<float>(d, lig[r], m, x.data(), y.data(), z.data());
-        energy<float>(J, prt[r], m, x.data(), y.data(), z.data(), false, o);
+        energy<float>(J, prt[r], m, x.data(), y.data(), z.data(), literal != 0, o);
         const float de = o.term[9] - en[(size_t)r * 10 + 9];
         const bool a = accept(u, de, d->minus_beta[tmp[r]]);
         acc[r] = a;
@@ -654,7 +656,7 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
         Out<float> o;
         const float *m = &mm[(size_t)part[r] * 6];
         transform<float>(d, lig[r], m, x.data(), y.data(), z.data());
-        energy<float>(J, prt[r], m, x.data(), y.data(), z.data(), false, o);
+        energy<float>(J, prt[r], m, x.data(), y.data(), z.data(), literal != 0, o);
         for (int i = 0; i < 10; ++i) tr[(size_t)r * 10 + i] = o.term[i];
       }
       // joint decisions from the pre-swap state, then the updates
@@ -703,7 +705,7 @@ int or_dock(const gd_job_desc *d, const gd_mc_param *mc, int nthreads, int rep_l
 int gd_cpu_dock(const gd_job_desc *d, const gd_mc_param *mc, gd_replica_out *out) {
   if (!d) return GD_ERR_ARG;
   return or_dock(d, mc, 0, 0, d->n_lig * d->n_prt * d->n_tmp, out, nullptr, nullptr, nullptr, 0,
-                 nullptr);
+                 nullptr, 1);
 }
 
 // Smallest float x with expf(x) > 0: the forced initial accept of the

@@ -29,7 +29,7 @@ class Oracle:
         L = self.lib
         L.or_energy.argtypes = [C.c_void_p, C.c_int, _ip, _fp, _fp, C.c_int, _dp, _dp, _dp, _fp, C.c_int]
         L.or_dock.argtypes = [C.c_void_p, C.POINTER(abi.gd_mc_param), C.c_int, C.c_int, C.c_int,
-                              C.POINTER(abi.gd_replica_out), _fp, _dp, _fp, C.c_int, _ip]
+                              C.POINTER(abi.gd_replica_out), _fp, _dp, _fp, C.c_int, _ip, C.c_int]
         L.or_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_uint32)]
         L.or_step_draws.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]
         L.or_exp_underflow_threshold.restype = C.c_float
@@ -52,9 +52,10 @@ class Oracle:
         assert rc == 0
         return terms, raw, scale, cout
 
-    def dock(self, job, mc, threads=0, rep_lo=0, rep_hi=0, trace=False, records=False):
+    def dock(self, job, mc, threads=0, rep_lo=0, rep_hi=0, trace=False, records=False, literal=False):
         """records=True adds (rec[n_rep][steps_total][17], count[n_rep]): the
-        accept records of the whole run."""
+        accept records of the whole run.  literal=True evaluates in the
+        reference's own FP32 expression order (else the GPU rounding)."""
         from geauxdock_amd.host import ReplicaOut
         n = job.n_rep
         out = ReplicaOut(n)
@@ -66,7 +67,7 @@ class Oracle:
         rc = self.lib.or_dock(C.byref(job.desc), C.byref(mc), threads, rep_lo, rep_hi, C.byref(out.c),
                               None if tr is None else tr.ctypes.data_as(_fp), C.byref(ms),
                               None if rec is None else rec.ctypes.data_as(_fp), cap,
-                              None if cnt is None else cnt.ctypes.data_as(_ip))
+                              None if cnt is None else cnt.ctypes.data_as(_ip), 1 if literal else 0)
         assert rc == 0
         if records:
             return out, tr, ms.value, (rec, cnt)

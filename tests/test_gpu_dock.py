@@ -277,11 +277,12 @@ def test_batch_matches_plain_bitwise(ene):
            for k in range(3)]
     plain = []
     for h, mc in zip(hjobs, mcs):
-        d = g.DeviceJob(h)
-        d.set_tiling(8, d.tiling()[1])
+        d = g.DeviceJob(h)   # the tiling shared launches give this complex
+        lc, pt, _ = g.host.tiling_lookup(h.desc.lig_natom, h.desc.prt_npoint, h.n_rep, True)
+        d.set_tiling(lc, pt)
         plain.append(d.dock(mc, stats=False)[0])
     devs = [g.DeviceJob(h) for h in hjobs]
-    for d in devs:
+    for d in devs:   # a tiling the batch must replace by the table's
         d.set_tiling(8, d.tiling()[1])
     for rep in range(3):  # several graph replays, eager work in between
         outs = _batch(devs, mcs)
@@ -292,6 +293,9 @@ def test_batch_matches_plain_bitwise(ene):
             assert np.array_equal(a.n_swap_lig, b.n_swap_lig)
             assert np.array_equal(a.n_swap_prt, b.n_swap_prt)
         devs[0].eval(np.zeros((1, 2), np.int32), np.zeros((1, 6), np.float32))
+    for h, d in zip(hjobs, devs):  # the batch ran every job at its table tiling
+        lc, pt, _ = g.host.tiling_lookup(h.desc.lig_natom, h.desc.prt_npoint, h.n_rep, True)
+        assert d.tiling() == (lc, pt)
 
 
 def test_batch_timed_path_matches_graph_path(ene):
@@ -411,12 +415,22 @@ def test_server_file_requests(ene, golden_job):
         g.lib().gd_server_destroy(s)
 
 
+def _shared_bytes(h):
+    """Device bytes of a complex at its shared-launch tiling."""
+    d = g.DeviceJob(h)
+    lc, pt, _ = g.host.tiling_lookup(h.desc.lig_natom, h.desc.prt_npoint, h.n_rep, True)
+    d.set_tiling(lc, pt)
+    b = g.lib().gd_job_device_bytes(d.h)
+    d.close()
+    return b
+
+
 def test_server_admission_budget(ene):
     """A request larger than the whole budget is refused; requests that fit
     only one at a time run one after another."""
     import ctypes as C
     h = g.synth_complex(800, 20, 2, 2, 2, seed=41, ene=ene)
-    foot = g.lib().gd_job_footprint(C.byref(h.desc))
+    foot = _shared_bytes(h)
     assert foot > 0
     cfg = abi.gd_server_config(hbm_budget=foot + foot // 2, max_active=8, sweep_steps=10)
     s = C.c_void_p()
@@ -461,3 +475,217 @@ def test_autotune_keeps_parity(ene, oracle):
     _, raw, coords = dev.eval(conf, pose, want_coords=True)
     _, r_raw, r_scale, _ = oracle.energy(job, conf, pose, mode=2, coords=coords)
     assert rel_err(raw, r_raw, r_scale).max() <= 1e-5
+
+
+def _server(**kw):
+    import ctypes as C
+    cfg = abi.gd_server_config(**kw)
+    s = C.c_void_p()
+    g.check(g.lib().gd_server_create(C.byref(cfg), C.byref(s)), "gd_server_create")
+    return s
+
+
+def _submit(s, h, mc, out=None):
+    import ctypes as C
+    t = C.c_int64()
+    g.check(g.lib().gd_server_submit(s, C.byref(h.desc), C.byref(mc), C.byref(out.c) if out else None,
+                                     C.byref(t)), "submit")
+    return t.value
+
+
+def _drain(s, results, want, max_steps=200):
+    import ctypes as C
+    for _ in range(max_steps):
+        na, nd = C.c_int(), C.c_int()
+        g.check(g.lib().gd_server_step(s, C.byref(na), C.byref(nd)), "step")
+        res = (abi.gd_job_result * 32)()
+        n = C.c_int()
+        g.lib().gd_server_results(s, res, 32, C.byref(n))
+        for i in range(n.value):
+            results[res[i].ticket] = (res[i].status, res[i].replica_steps, res[i].t_submit,
+                                      res[i].t_admit, res[i].t_done)
+        if len(results) >= want:
+            return
+    raise AssertionError("server did not finish: %d of %d" % (len(results), want))
+
+
+def test_server_charges_device_bytes(ene):
+    """While jobs are active the server's usage is the sum of their actual
+    device bytes (at the shared-launch tiling); when all retire it is 0."""
+    import ctypes as C
+    hs = [g.synth_complex(P, L, 2, 2, 2, seed=s, ene=ene) for P, L, s in [(700, 14, 61), (1500, 33, 62),
+                                                                         (2600, 70, 63)]]
+    want = sum(_shared_bytes(h) for h in hs)
+    s = _server(hbm_budget=64 << 30, max_active=8, sweep_steps=10)
+    try:
+        for k, h in enumerate(hs):
+            _submit(s, h, g.make_mc(40, seed=1, job_id=k))
+        na, nd = C.c_int(), C.c_int()
+        g.check(g.lib().gd_server_step(s, C.byref(na), C.byref(nd)), "step")
+        used, pend = C.c_size_t(), C.c_int()
+        g.lib().gd_server_usage(s, C.byref(used), C.byref(pend))
+        assert na.value == 3 and pend.value == 0 and used.value == want, (used.value, want)
+        res = {}
+        _drain(s, res, 3)
+        g.lib().gd_server_usage(s, C.byref(used), C.byref(pend))
+        assert used.value == 0 and all(v[0] == 0 for v in res.values())
+    finally:
+        g.lib().gd_server_destroy(s)
+
+
+def test_server_cancel(ene):
+    """Cancelling a pending request, an active job and a file request (queued
+    or already with the loader thread) each yields one GD_ERR_STATE result and
+    nothing runs on; an unknown ticket is an argument error."""
+    import ctypes as C
+    import os
+    from tests.conftest import GOLDEN_DATA as d
+    h = g.synth_complex(900, 20, 2, 2, 2, seed=71, ene=ene)
+    s = _server(hbm_budget=64 << 30, max_active=1, sweep_steps=10)
+    try:
+        ta = _submit(s, h, g.make_mc(200, seed=1, job_id=1))
+        tb = _submit(s, h, g.make_mc(200, seed=1, job_id=2))
+        na, nd = C.c_int(), C.c_int()
+        g.check(g.lib().gd_server_step(s, C.byref(na), C.byref(nd)), "step")   # a active, b pending
+        req = abi.gd_request_files(os.path.join(d, "1a07C1.sdf").encode(),
+                                   os.path.join(d, "1a07C1-0.8.ff").encode(),
+                                   os.path.join(d, "1a07C-1.pdb").encode(), b"1a07C1",
+                                   C.pointer(ene), 1, 0.044, 0.036)
+        tc = C.c_int64()
+        g.check(g.lib().gd_server_submit_files(s, C.byref(req), C.byref(g.make_mc(50, seed=2)),
+                                               C.byref(tc)), "submit_files")
+        assert g.lib().gd_server_cancel(s, tc.value) == abi.GD_OK
+        assert g.lib().gd_server_cancel(s, tb) == abi.GD_OK
+        assert g.lib().gd_server_cancel(s, ta) == abi.GD_OK
+        assert g.lib().gd_server_cancel(s, 987654) == abi.GD_ERR_ARG
+        res = {}
+        import time
+        for _ in range(100):
+            _drain(s, res, 0, max_steps=1)
+            if len(res) == 3:
+                break
+            time.sleep(0.02)   # a file request cancelled while being parsed reports when parsed
+        assert set(res) == {ta, tb, tc.value}, res
+        assert all(v[0] == abi.GD_ERR_STATE for v in res.values()), res
+        assert res[ta][1] > 0 and res[tb][1] == 0
+        n_rep = C.c_int()
+        assert g.lib().gd_server_take(s, tc, C.byref(n_rep), None) == abi.GD_ERR_ARG
+        used, pend = C.c_size_t(), C.c_int()
+        g.lib().gd_server_usage(s, C.byref(used), C.byref(pend))
+        assert used.value == 0 and pend.value == 0
+    finally:
+        g.lib().gd_server_destroy(s)
+
+
+def test_server_retires_converged_jobs(ene):
+    """With convergence retirement a job leaves after its third sweep when the
+    tolerance cannot be missed (3 x 10 steps instead of 200), and its result is
+    the same alone and among other jobs."""
+    h = g.synth_complex(800, 20, 2, 2, 2, seed=81, ene=ene)
+    mc = g.make_mc(200, steps_per_exchange=20, move_mode="gaussian", seed=5, job_id=9,
+                   axes=("temp", "lig", "prt"))
+    outs = []
+    for others in (0, 2):
+        s = _server(hbm_budget=64 << 30, max_active=8, sweep_steps=10, converge_sweeps=2,
+                    converge_tol=1e9)
+        try:
+            for k in range(others):
+                _submit(s, g.synth_complex(1200, 30, 2, 2, 2, seed=82 + k, ene=ene),
+                        g.make_mc(500, seed=3, job_id=k))
+            o = g.host.ReplicaOut(h.n_rep)
+            t = _submit(s, h, mc, o)
+            res = {}
+            _drain(s, res, others + 1)
+            assert res[t][0] == 0 and res[t][1] == h.n_rep * 30, res[t]
+            outs.append(o)
+        finally:
+            g.lib().gd_server_destroy(s)
+    assert np.array_equal(outs[0].energy, outs[1].energy)
+    assert np.array_equal(outs[0].movematrix, outs[1].movematrix)
+
+
+def test_server_submit_does_not_wait_for_a_sweep(ene):
+    """A request submitted while gd_server_step runs a long sweep returns at
+    once and is stamped on arrival, before that sweep ends."""
+    import ctypes as C
+    import threading
+    import time
+    big = g.synth_complex(3000, 40, 20, 11, 8, seed=2, ene=ene)
+    small = g.synth_complex(600, 12, 2, 2, 1, seed=91, ene=ene)
+    s = _server(hbm_budget=64 << 30, max_active=8, sweep_steps=3000)
+    try:
+        _submit(s, big, g.make_mc(3000, seed=1, job_id=1))
+        clock = g.lib().gd_server_clock
+        clock.restype = C.c_double
+        span = {}
+
+        def run():
+            na, nd = C.c_int(), C.c_int()
+            span["t0"] = clock(s)
+            g.check(g.lib().gd_server_step(s, C.byref(na), C.byref(nd)), "step")
+            span["t1"] = clock(s)
+
+        th = threading.Thread(target=run)
+        th.start()
+        time.sleep(0.05)
+        w0 = time.perf_counter()
+        t = _submit(s, small, g.make_mc(10, seed=2, job_id=2))
+        dt = time.perf_counter() - w0
+        th.join()
+        res = {}
+        _drain(s, res, 2)
+        assert span["t1"] - span["t0"] > 0.2, span   # the sweep was long
+        assert dt < 0.05, dt
+        assert res[t][2] < span["t1"], (res[t], span)
+    finally:
+        g.lib().gd_server_destroy(s)
+
+
+def test_tiling_is_a_function_of_the_job(tmp_path):
+    """A job's tiling, and so its bits, is the same in a fresh process and in
+    one that first autotuned another job of the same size bucket."""
+    import subprocess
+    import sys
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = r"""
+import sys, ctypes as C, numpy as np
+sys.path.insert(0, %r)
+import geauxdock_amd as g
+from geauxdock_amd import abi
+g.check(g.lib().gd_init(0), "init")
+ene = g.load_enepara()
+if sys.argv[1] == "tune":
+    other = g.DeviceJob(g.synth_complex(2200, 33, 4, 4, 2, seed=7, ene=ene))
+    t = abi.gd_tiling(); ms = C.c_double()
+    g.check(g.lib().gd_autotune(other.h, 3, C.byref(t), C.byref(ms)), "tune")
+dev = g.DeviceJob(g.synth_complex(2100, 35, 4, 4, 2, seed=8, ene=ene))
+out, _ = dev.dock(g.make_mc(40, seed=3))
+np.save(sys.argv[2], np.concatenate([out.energy.ravel(), out.movematrix.ravel(),
+                                     np.array(dev.tiling(), np.float32)]))
+""" % root
+    res = []
+    for mode in ("plain", "tune"):
+        f = str(tmp_path / (mode + ".npy"))
+        r = subprocess.run([sys.executable, "-c", code, mode, f], capture_output=True, text=True,
+                           timeout=240)
+        assert r.returncode == 0, r.stderr
+        res.append(np.load(f))
+    assert np.array_equal(res[0], res[1])
+
+
+def test_autotune_shared(ene):
+    """The shared-launch sweep returns a candidate of its grid and leaves the
+    jobs at the table's tiling."""
+    import ctypes as C
+    hs = [g.synth_complex(P, L, 2, 3, 2, seed=s, ene=ene) for P, L, s in [(1800, 20, 95), (2500, 44, 96)]]
+    devs = [g.DeviceJob(h) for h in hs]
+    arr = (C.c_void_p * len(devs))(*[d.h.value for d in devs])
+    t = abi.gd_tiling()
+    ms = C.c_double()
+    g.check(g.lib().gd_autotune_shared(arr, len(devs), 4, C.byref(t), C.byref(ms)), "tune shared")
+    assert t.atoms_per_lane == 4 and t.tile_points in (32, 40, 48, 64, 80, 96, 128)
+    assert t.waves_per_cu in (16, 24, 32) and ms.value > 0
+    for h, d in zip(hs, devs):
+        lc, pt, _ = g.host.tiling_lookup(h.desc.lig_natom, h.desc.prt_npoint, h.n_rep, True)
+        assert d.tiling() == (lc, pt)

@@ -400,7 +400,8 @@ def test_swap_off_leaves_chain(ene, oracle):
 
 def test_cpu_dock_entry_point(golden_job, oracle):
     """gd_cpu_dock (the CPU backend's Dock in C-ABI form): the initial pass of the
-    golden complex reproduces the published line; a short run equals or_dock."""
+    golden complex reproduces the published line; a short run equals or_dock in
+    the reference's own FP32 expression order."""
     from geauxdock_amd import abi
     from geauxdock_amd.host import ReplicaOut
     L = oracle.lib
@@ -413,7 +414,7 @@ def test_cpu_dock_entry_point(golden_job, oracle):
     mc = g.make_mc(5, move_mode="gaussian", seed=9)
     out = ReplicaOut(golden_job.n_rep)
     assert L.gd_cpu_dock(C.byref(golden_job.desc), C.byref(mc), C.byref(out.c)) == 0
-    ref, _, _ = oracle.dock(golden_job, mc, threads=2)
+    ref, _, _ = oracle.dock(golden_job, mc, threads=2, literal=True)
     assert np.array_equal(out.energy, ref.energy) and np.array_equal(out.n_accept, ref.n_accept)
 
 
