@@ -490,7 +490,7 @@ def traffic_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out_js
     n = 256 << 20
     cf = _counter_means(copy_fetch_dir, "FETCH_SIZE")
     cw = _counter_means(copy_write_dir, "WRITE_SIZE")
-    copy_k = [k for k in cf if "copyBuffer" in k or "elementwise" in k]
+    copy_k = [k for k in cf if "copy" in k.lower() or "elementwise" in k]
     kf = max(copy_k, key=lambda k: cf[k])          # the copy: the largest reader
     fetch_scale = n / (cf[kf] * 1024.0)
     kw = kf if kf in cw else max(cw, key=lambda k: cw[k])
@@ -511,18 +511,16 @@ def traffic_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out_js
 
 
 def copy_calibration(n_bytes=256 << 20, reps=3):
-    """A plain device-to-device copy of known size: the HBM counters
-    (FETCH_SIZE, WRITE_SIZE) of its dispatches against n_bytes give the scale
-    of those counters on this device."""
-    import torch
-    x = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    for _ in range(reps):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    return {"workload": "copy-calibration", "bytes_read_per_copy": n_bytes,
-            "bytes_written_per_copy": n_bytes, "copies": reps}
+    """A plain device-to-device copy of known size (the library's own 16-byte
+    vector copy kernel, gd_copy_kernel): the HBM counters (FETCH_SIZE,
+    WRITE_SIZE) of its dispatches against n_bytes give the scale of those
+    counters on this device."""
+    import geauxdock_amd as g
+    ms = C.c_double()
+    g.check(g.lib().gd_device_copy(n_bytes, reps, C.byref(ms)), "gd_device_copy")
+    return {"workload": "copy-calibration", "kernel": "gd_copy_kernel", "bytes_read_per_copy": n_bytes,
+            "bytes_written_per_copy": n_bytes, "copies": reps, "ms_per_copy": ms.value,
+            "gb_per_s": 2 * n_bytes / (ms.value / 1000.0) / 1e9}
 
 
 def main():
@@ -558,12 +556,12 @@ def main():
               "(jobs shard across GPUs without communication)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
-    if args.workload == "copy":
-        print(json.dumps(copy_calibration()), flush=True)
-        return
     import geauxdock_amd as g
     from geauxdock_amd import abi
     g.check(g.lib().gd_init(0), "gd_init")
+    if args.workload == "copy":
+        print(json.dumps(copy_calibration()), flush=True)
+        return
     if args.workload == "c1":
         print(json.dumps(c1_extra()), flush=True)
         return

@@ -168,6 +168,7 @@ PROTOTYPES = {
                                 C.c_int, C.c_int, _fp]),
     "gd_move_incr_words": (C.c_int, [C.c_int, _fp, C.c_int, C.POINTER(C.c_uint32), _fp]),
     "gd_measure_sclk": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
+    "gd_device_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "gd_far_r2": (C.c_float, [C.c_float, C.c_float]),
     "gd_near_r2": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
 }

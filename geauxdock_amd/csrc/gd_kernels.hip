@@ -1172,3 +1172,12 @@ __global__ __launch_bounds__(64) void gd_clock_kernel(long long ticks, float see
   }
   if (a == 12345.0f) sink[0] = a;  // keeps the FMA chain
 }
+
+// ---------------------------------------------------------------------------
+// Plain device copy (n16 16-byte vectors, grid-stride): the known-size
+// reference against which HBM traffic counters are scaled.
+__global__ __launch_bounds__(256) void gd_copy_kernel(const float4 *__restrict__ src,
+                                                      float4 *__restrict__ dst, size_t n16) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256)
+    dst[i] = src[i];
+}

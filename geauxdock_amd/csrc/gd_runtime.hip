@@ -36,6 +36,7 @@ __global__ void gd_finish_kernel(const GdJob *, const GdBlock *, int, int, const
 __global__ void gd_exchange_kernel(const GdJob *, const GdExch *);
 __global__ void gd_move_draws_kernel(uint64_t, uint32_t, int, GdScale6, int, uint32_t, int, float *);
 __global__ void gd_clock_kernel(long long, float, long long *, float *);
+__global__ void gd_copy_kernel(const float4 *, float4 *, size_t);
 __global__ void gd_swap_propose_kernel(const GdJob *, const GdBlock *, int, const int *);
 __global__ void gd_swap_accept_kernel(const GdJob *, const GdBlock *, int, const int *, int,
                                       const int *);
@@ -1403,6 +1404,38 @@ gd_status gd_measure_sclk(int ms, double *sclk_mhz) {
   if (e != hipSuccess) return hip_fail(e, "gd_clock_kernel");
   if (h[1] <= 0) return GD_ERR_STATE;
   *sclk_mhz = (double)h[0] / (double)h[1] * (double)wall_khz / 1000.0;
+  return GD_OK;
+}
+
+// A plain device-to-device copy of `bytes` (a multiple of 16), `reps` times:
+// the calibration for HBM traffic counters (each dispatch reads and writes
+// exactly `bytes`).  *ms: mean time per copy.
+gd_status gd_device_copy(size_t bytes, int reps, double *ms) {
+  if (bytes == 0 || bytes % 16 != 0 || reps <= 0) return GD_ERR_ARG;
+  gd_status st = ensure_device();
+  if (st != GD_OK) return st;
+  void *a = nullptr, *b = nullptr;
+  hipError_t e = hipMalloc(&a, bytes);
+  if (e == hipSuccess) e = hipMalloc(&b, bytes);
+  if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+  if (e != hipSuccess) {
+    if (a) hipFree(a);
+    return hip_fail(e, "gd_device_copy buffers");
+  }
+  const size_t n16 = bytes / 16;
+  const int grid = 4 * 8 * (g_cu > 0 ? g_cu : 256);
+  e = hipDeviceSynchronize();
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int r = 0; r < reps && e == hipSuccess; ++r) {
+    hipLaunchKernelGGL(gd_copy_kernel, dim3(grid), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n16);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  const auto t1 = std::chrono::steady_clock::now();
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return hip_fail(e, "gd_copy_kernel");
+  if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / reps;
   return GD_OK;
 }
 

@@ -381,6 +381,10 @@ gd_status gd_move_incr_words(int move_mode, const float move_scale[6], int n, co
 /* Engine clock under an all-CU FMA load, in MHz, measured over about `ms`
  * milliseconds (1..2000) from the shader-clock and constant-rate counters. */
 gd_status gd_measure_sclk(int ms, double *sclk_mhz);
+/* A plain device-to-device copy of `bytes` (a multiple of 16), `reps` times,
+ * by a grid-stride 16-byte vector kernel: the known-size calibration of HBM
+ * traffic counters.  *ms (optional): mean milliseconds per copy. */
+gd_status gd_device_copy(size_t bytes, int reps, double *ms);
 /* The pair kernel's far-field and near-branch r^2 thresholds of a
  * (ligand type, point type) parameter row (tests of the exact-zero shortcut). */
 float gd_far_r2(float pmf0, float el1);
