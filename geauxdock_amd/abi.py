@@ -121,6 +121,7 @@ PROTOTYPES = {
     "gd_job_sizes": (C.c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
     "gd_job_device_bytes": (C.c_size_t, [_vp]),
     "gd_energy_eval": (C.c_int, [_vp, C.c_int, _ip, _fp, _fp, _fp, _fp]),
+    "gd_energy_eval_noalloc": (C.c_int, [_vp, C.c_int, _ip, _fp, _fp, _fp]),
     "gd_dock": (C.c_int, [_vp, C.POINTER(gd_mc_param), C.POINTER(gd_replica_out),
                           C.POINTER(gd_stats)]),
     "gd_dock_records": (C.c_int, [_vp, C.c_int, _fp, _ip]),

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgeauxdock_amd.so")
-SOURCES = ["gd_kernels.hip", "gd_runtime.hip", "gd_batch.hip", "gd_dropin.cpp", "gd_host.cpp"]
+SOURCES = ["gd_kernels.hip", "gd_runtime.hip", "gd_batch.hip", "gd_noalloc.hip", "gd_dropin.cpp",
+           "gd_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -185,6 +185,21 @@ size_t gd_job_device_bytes(const gd_job *job);
 gd_status gd_energy_eval(gd_job *job, int n_pose, const int *conf, const float *pose,
                          float *terms, float *raw, float *coords);
 
+/* gd_energy_eval that allocates nothing: no hipMalloc, hipFree or other
+ * allocating call, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  No counterpart in
+ * the reference.  The kernels' scratch (partial sums, staged poses, work
+ * items, results) lives in one static device array of
+ * GD_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; poses run in
+ * chunks of 64-pose blocks that fit it, and a job whose single block needs
+ * more returns GD_ERR_SIZE.  Same results, bit for bit, as gd_energy_eval
+ * (terms[n][10], raw[n][9] optional; no coordinates).  The job's replica
+ * state is not touched.  Calls that overlap on one GPU share the array and
+ * race; the array keeps the last call's contents. */
+#define GD_NOALLOC_SCRATCH_BYTES (64 << 20)
+gd_status gd_energy_eval_noalloc(gd_job *job, int n_pose, const int *conf, const float *pose,
+                                 float *terms, float *raw);
+
 /* Plain one-job docking: initial-energy pass, then steps_total MC steps with
  * temperature exchange, replica state resident on the device. */
 gd_status gd_dock(gd_job *job, const gd_mc_param *mc, gd_replica_out *out, gd_stats *stats);
