@@ -510,6 +510,61 @@ def traffic_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out_js
     return res
 
 
+def trace_summary(trace_csv):
+    """Per-kernel dispatch counts and mean durations from a rocprofv3
+    kernel-trace CSV, and the pair kernel's per-step figure to set against
+    bench.py's event-timed mean: for one-stream steps the dispatch duration;
+    for two-stream steps (two half-dispatches on different streams, launched
+    back to back) the span from the first start to the last end of each pair
+    of halves.  Whole-job pair dispatches of the exchange points' swap passes
+    are reported apart."""
+    import csv
+    rows = list(csv.DictReader(open(trace_csv)))
+    kern = {}
+    for r in rows:
+        d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        k = kern.setdefault(r["Kernel_Name"], [0, 0.0])
+        k[0] += 1
+        k[1] += d
+    out = {"kernels": {k: {"dispatches": v[0], "mean_us": v[1] / v[0] / 1000.0} for k, v in kern.items()}}
+    pair = sorted((r for r in rows if "gd_pair_kernel<" in r["Kernel_Name"]),
+                  key=lambda r: int(r["Correlation_Id"]))
+    if pair:
+        grids = sorted({int(r["Grid_Size_X"]) for r in pair})
+        full = max(grids)
+        halves = [r for r in pair if int(r["Grid_Size_X"]) != full]
+        whole = [r for r in pair if int(r["Grid_Size_X"]) == full]
+        spans = []
+        i = 0
+        while i + 1 < len(halves):
+            a, b = halves[i], halves[i + 1]
+            if a["Stream_Id"] != b["Stream_Id"] and int(a["Grid_Size_X"]) + int(b["Grid_Size_X"]) == full:
+                spans.append(max(int(a["End_Timestamp"]), int(b["End_Timestamp"])) -
+                             min(int(a["Start_Timestamp"]), int(b["Start_Timestamp"])))
+                i += 2
+            else:
+                i += 1
+        dur = lambda rs: sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in rs) / max(1, len(rs))
+        # steady-state step period: successive starts of the first half's
+        # dispatches (the stream of the first half-dispatch), gaps over 3x the
+        # median (run and warm-up boundaries) left out
+        period = None
+        if halves:
+            sa = halves[0]["Stream_Id"]
+            starts = sorted(int(r["Start_Timestamp"]) for r in halves if r["Stream_Id"] == sa)
+            d = [b - a for a, b in zip(starts, starts[1:])]
+            if d:
+                med = sorted(d)[len(d) // 2]
+                keep = [x for x in d if x <= 3 * med]
+                period = sum(keep) / len(keep) / 1000.0
+        out["pair"] = {"whole_job_dispatches": len(whole), "whole_job_mean_us": dur(whole) / 1000.0,
+                       "two_stream_steps": len(spans),
+                       "two_stream_step_span_mean_us": (sum(spans) / len(spans) / 1000.0) if spans else None,
+                       "half_dispatch_mean_us": dur(halves) / 1000.0 if halves else None,
+                       "two_stream_step_period_us": period}
+    return out
+
+
 def copy_calibration(n_bytes=256 << 20, reps=3):
     """A plain device-to-device copy of known size (the library's own 16-byte
     vector copy kernel, gd_copy_kernel): the HBM counters (FETCH_SIZE,
@@ -536,6 +591,8 @@ def main():
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                     help="summarise rocprofv3 FETCH_SIZE/WRITE_SIZE runs into "
                          "profiles/pair_kernel_c2_traffic.json (no GPU needed)")
+    ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
+                    help="summarise a rocprofv3 --kernel-trace CSV (no GPU needed)")
     ap.add_argument("--jobs", type=int, default=256, help="c3/c4/c5: jobs (requests)")
     ap.add_argument("--mc-steps", type=int, default=5000, help="c3/c4: MC steps per job; c5: "
                     "the step scale is mc_steps / 10000 of the 2,000-10,000 drawn per request")
@@ -547,6 +604,9 @@ def main():
                     help="default run: the headline line only, without its c1/c3 keys")
     args = ap.parse_args()
     host_cores()   # the CPU topology, before any OpenMP region binds this thread
+    if args.trace_summary:
+        print(json.dumps(trace_summary(args.trace_summary), indent=1))
+        return
     if args.traffic:
         print(json.dumps(traffic_summary(*args.traffic, os.path.join(ROOT, "profiles",
                                                                       "pair_kernel_c2_traffic.json"))))
@@ -595,8 +655,22 @@ def main():
     sclk_after = C.c_double()
     g.check(g.lib().gd_measure_sclk(200, C.byref(sclk_after)), "gd_measure_sclk")
     value, jobs_per_s = c2_rates(job.n_rep, steps, wall)
-    pair_ms = st.ms_pair_kernel / max(1, st.n_pair_launch)
-    achieved = flops_step / (pair_ms / 1000.0) / 1e12
+    # Two streams step the two halves of the replica blocks, so a step's two
+    # pair half-launches overlap each other and the other half's reduce and
+    # finish kernels: the pair kernel's rate is its flops per MC step over the
+    # steady-state step period (HIP events at the pair launches of every 16th
+    # step, on the pair kernel's stream), which charges the whole step -- tail
+    # kernels and exchange points included -- to the pair kernel.
+    period_ms = st.ms_step_period
+    achieved = flops_step / (period_ms / 1000.0) / 1e12
+    half_ms = st.ms_pair_kernel / max(1, st.n_pair_launch)
+    # the same kernel alone: one stream, one launch per step, event-timed
+    dev.set_streams(1)
+    _, st1 = dev.dock(g.make_mc(400, steps_per_dump=400, steps_per_exchange=STEPS_PER_EXCHANGE,
+                                move_mode="gaussian", seed=SEED, axes=AXES), stats=True)
+    dev.set_streams(2)
+    iso_ms = st1.ms_pair_kernel / max(1, st1.n_pair_launch)
+    iso = flops_step / (iso_ms / 1000.0) / 1e12
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "pair_kernel_c2_traffic.json")
     if os.path.exists(tfile):
@@ -644,9 +718,17 @@ def main():
             "kernel": "gd_pair_kernel<%d>" % dev.tiling()[0],
             "flops_per_launch": flops_step,
             "prt_pairs_per_launch": prt_pairs,
-            "mean_launch_ms": pair_ms,
-            "launches_timed": st.n_pair_launch,
-            "prt_only_tflops": PRT_FLOPS * prt_pairs / (pair_ms / 1000.0) / 1e12,
+            "basis": "one launch = one MC step's pair work, two half-launches on two streams; its "
+                     "duration = the steady-state MC step period (events at the pair launches of every "
+                     "16th step on the pair kernel's stream), the whole step charged to the pair kernel",
+            "mean_launch_ms": period_ms,
+            "launches_timed": st.n_pair_launch // 2,
+            "half_launch_mean_ms": half_ms,
+            "pair_span_mean_ms": st.ms_pair_span,
+            "prt_only_tflops": PRT_FLOPS * prt_pairs / (period_ms / 1000.0) / 1e12,
+            "isolated": {"streams": 1, "mean_launch_ms": iso_ms, "achieved": iso,
+                         "frac": iso / PEAK_FP32_TFLOPS, "launches_timed": st1.n_pair_launch,
+                         "note": "the same launch alone on one stream, 400 MC steps after the timed region"},
         },
     }
     # the same job in the reference's plain shape: dumps of 200 steps, each

@@ -75,6 +75,7 @@ class gd_stats(C.Structure):
         ("ms_total", C.c_double), ("ms_pair_kernel", C.c_double),
         ("ms_finish_kernel", C.c_double), ("n_pair_launch", C.c_int),
         ("pair_evals", C.c_double), ("replica_steps", C.c_double),
+        ("ms_step_period", C.c_double), ("ms_pair_span", C.c_double),
     ]
 
 
@@ -134,6 +135,7 @@ PROTOTYPES = {
     "gd_batch_dock": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(gd_mc_param),
                                 C.POINTER(gd_replica_out), C.POINTER(gd_stats)]),
     "gd_tiling_get": (C.c_int, [_vp, C.POINTER(gd_tiling)]),
+    "gd_job_set_streams": (C.c_int, [_vp, C.c_int]),
     "gd_tiling_set": (C.c_int, [_vp, C.POINTER(gd_tiling)]),
     "gd_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(gd_tiling), C.POINTER(C.c_double)]),
     "gd_autotune_shared": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.POINTER(gd_tiling),

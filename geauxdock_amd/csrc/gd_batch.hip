@@ -371,6 +371,9 @@ gd_status batch_dock_impl(gd_job **jobs_in, int n_jobs, const gd_mc_param *mc, g
     stats->n_pair_launch = tp.n;
     stats->pair_evals = pair_evals;
     stats->replica_steps = rep_steps;
+    // the batch's timed runs synchronise every 64 steps: the period on the host clock
+    stats->ms_step_period = steps_max > 0 ? stats->ms_total / steps_max : 0.0;
+    stats->ms_pair_span = tp.n ? tp.ms / tp.n : 0.0;
   }
   if (out)
     for (int k = 0; k < n_jobs; ++k)

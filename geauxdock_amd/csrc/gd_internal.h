@@ -30,6 +30,12 @@ struct gd_job {
   GdItem *d_items = nullptr;
   GdBlock *d_blocks = nullptr;
   int n_items = 0, n_blocks = 0;
+  // the plain path's two stepping halves: blocks [0, nb_half[0]) and the rest,
+  // and their items (the job's items of those blocks, in the job's order)
+  GdItem *d_items_half[2] = {nullptr, nullptr};
+  int n_items_half[2] = {0, 0}, nb_half[2] = {0, 0};
+  int n_streams = 2;         // streams the plain path steps on (1 or 2)
+  hipStream_t stream2 = nullptr;
   // protein points in source order, laid out on the device per tile count
   std::vector<float4> src_pco;     // [n_prt][P]
   std::vector<float4> src_pattr;   // [P]
@@ -83,6 +89,7 @@ struct KTimer {
   int used = 0;
   double ms = 0.0;
   int n = 0;
+  double first_to_last_ms = 0.0;  // first to last timed launch start (collect)
   ~KTimer();
   gd_status begin(hipStream_t s);   // records start event
   gd_status end(hipStream_t s);     // records stop event

@@ -37,6 +37,13 @@
 #define GD_EXP_UNDERFLOW (-103.97207641601562f)
 
 __device__ __forceinline__ int gd_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+// Lane id within the wave, computed afresh (volatile: never reused from an
+// earlier value, so a lane-derived index need not stay live across a loop).
+__device__ __forceinline__ int gd_lane_id() {
+  int l;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+  return l;
+}
 __device__ __forceinline__ float gd_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float gd_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float gd_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
@@ -429,6 +436,10 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   const float inv_el1 = k.inv_el1;
   float evdw = 0.0f, epmf = 0.0f, epsp = 0.0f, ehdb = 0.0f;
   gd_pair_points<LC>(J, b, pc, k, lx, ly, lz, trow, ele, hpc, evdw, epmf, epsp, ehdb);
+  // the lane's replica for the stores below, derived again rather than kept
+  // live through the point loop (where it would take two VGPRs)
+  const int lane_e = gd_lane_id();
+  const int r_e = it.rb + lane_e;
 
   // KDE: this tile's share of every chunk atom's anchor bin, a contiguous range
   // [f(b), f(b+1)) with f(b) = floor(n b / nP) evaluated in FP32: f is
@@ -472,7 +483,7 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
           d += __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         }
       }
-      pm[((size_t)m * nL + it.a) * Rp + r] = d;
+      pm[((size_t)m * nL + it.a) * Rp + r_e] = d;
     }
   }
 
@@ -482,14 +493,14 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   for (int j = 0; j < LC; ++j) eele = __builtin_fmaf(atoms[l0 + j].q, ele[j], eele);  // padding: q = 0
   eele *= inv_el1;
   // group reduction: the group's tiles summed in wave order
-  red[wave][0][lane] = evdw;
-  red[wave][1][lane] = eele;
-  red[wave][2][lane] = epmf;
-  red[wave][3][lane] = epsp;
-  red[wave][4][lane] = ehdb;
-  red[wave][5][lane] = ekde;
+  red[wave][0][lane_e] = evdw;
+  red[wave][1][lane_e] = eele;
+  red[wave][2][lane_e] = epmf;
+  red[wave][3][lane_e] = epsp;
+  red[wave][4][lane_e] = ehdb;
+  red[wave][5][lane_e] = ekde;
 #pragma unroll
-  for (int j = 0; j < LC; ++j) red[wave][GD_NLIN + j][lane] = hpc[j];
+  for (int j = 0; j < LC; ++j) red[wave][GD_NLIN + j][lane_e] = hpc[j];
   __syncthreads();
   const int nG = J.nG;
   const size_t nS = (size_t)J.nL * nG;
@@ -497,13 +508,13 @@ __device__ __forceinline__ void gd_pair_item(const GdJob *__restrict__ jobs, con
   GD_AS_GLOBAL float *__restrict__ pl = gd_g(J.part_lin);
   GD_AS_GLOBAL float *__restrict__ ph = gd_g(J.part_hpc);
   for (int q = wave; q < GD_NLIN + LC; q += GD_GROUP) {
-    float v = red[0][q][lane];
+    float v = red[0][q][lane_e];
 #pragma unroll
-    for (int w = 1; w < GD_GROUP; ++w) v += red[w][q][lane];
+    for (int w = 1; w < GD_GROUP; ++w) v += red[w][q][lane_e];
     if (q < GD_NLIN)
-      pl[((size_t)q * nS + slot) * Rp + r] = v;
+      pl[((size_t)q * nS + slot) * Rp + r_e] = v;
     else if (q - GD_NLIN < nl)
-      ph[((size_t)(l0 + q - GD_NLIN) * nG + it.b) * Rp + r] = v;
+      ph[((size_t)(l0 + q - GD_NLIN) * nG + it.b) * Rp + r_e] = v;
   }
   __syncthreads();  // red[] is reused by the next item of a persistent group
 }

@@ -288,6 +288,10 @@ class DeviceJob:
         check(lib().gd_tiling_get(self.h, C.byref(t)), "gd_tiling_get")
         return t.atoms_per_lane, t.tile_points
 
+    def set_streams(self, n):
+        """Streams gd_dock steps this job on (1 or 2)."""
+        check(lib().gd_job_set_streams(self.h, n), "gd_job_set_streams")
+
     def set_tiling(self, atoms_per_lane, tile_points):
         t = abi.gd_tiling(atoms_per_lane, tile_points, 0)
         check(lib().gd_tiling_set(self.h, C.byref(t)), "gd_tiling_set")

@@ -148,11 +148,18 @@ typedef struct {
 
 typedef struct {
   double ms_total;          /* wall time of the MC loop (host clock) */
-  double ms_pair_kernel;    /* summed HIP-event time of the timed pair-energy launches */
+  double ms_pair_kernel;    /* summed HIP-event time of the timed pair-energy launches (on two
+                               streams: of both half-launches of each timed step) */
   double ms_finish_kernel;  /* reserved, 0: the tail kernels are not event-timed */
-  int n_pair_launch;        /* pair-kernel launches timed: those of MC steps 0, 16, 32, ... */
+  int n_pair_launch;        /* pair-kernel launches timed (steps 0, 16, 32, ...; two per step on
+                               two streams) */
   double pair_evals;        /* ligand-atom x protein-point evaluations done */
   double replica_steps;     /* replica x MC steps done (initial pass excluded) */
+  double ms_step_period;    /* steady-state MC step period: from the first to the last timed
+                               step's pair launch start (HIP events on the pair kernel's
+                               stream) over the steps between */
+  double ms_pair_span;      /* mean time from the first start to the last end of a timed step's
+                               pair launches (the launch itself on one stream) */
 } gd_stats;
 
 typedef struct gd_job gd_job;         /* device-resident complex */
@@ -262,6 +269,11 @@ typedef struct {
   int waves_per_cu;         /* persistent grid size per CU */
 } gd_tiling;
 gd_status gd_tiling_get(const gd_job *job, gd_tiling *t);
+/* Streams gd_dock steps the job on (1 or 2, default 2): with two, each steps
+ * half of the replica blocks between exchange points, so one half's reduce
+ * and finish kernels and launch tail overlap the other half's pair work.  The
+ * results are the same bits either way. */
+gd_status gd_job_set_streams(gd_job *job, int n_streams);
 gd_status gd_tiling_set(gd_job *job, const gd_tiling *t);
 /* Sweep atoms per lane x protein tile counts on `job` for `steps` MC steps
  * each and apply the fastest to this job only (the reference's counterpart,
