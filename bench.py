@@ -471,16 +471,24 @@ def serve(rates, n_jobs, step_scale):
             "plain_note": "same requests, one job per launch sequence, back to back"}
 
 
-def _counter_means(run_dir, counter):
-    """Mean per-dispatch value of one counter, by kernel name, from a
-    rocprofv3 --pmc --output-format csv directory."""
+def _counter_rows(run_dir, counter):
+    """(kernel name, grid size, value) of every dispatch's reading of one
+    counter, from a rocprofv3 --pmc --output-format csv directory."""
     import csv
     import glob
-    vals = {}
+    rows = []
     for f in glob.glob(os.path.join(run_dir, "*counter_collection.csv")):
         for r in csv.DictReader(open(f)):
             if r["Counter_Name"] == counter:
-                vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+                rows.append((r["Kernel_Name"], int(r.get("Grid_Size") or 0), float(r["Counter_Value"])))
+    return rows
+
+
+def _counter_means(run_dir, counter):
+    """Mean per-dispatch value of one counter, by kernel name."""
+    vals = {}
+    for k, _, v in _counter_rows(run_dir, counter):
+        vals.setdefault(k, []).append(v)
     return {k: sum(v) / len(v) for k, v in vals.items()}
 
 
@@ -502,9 +510,31 @@ def traffic_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, out_js
     for k in f:
         res["kernels"][k] = {"fetch_kib": f[k], "write_kib": w.get(k, 0.0),
                              "hbm_bytes": f[k] * 1024.0 * fetch_scale + w.get(k, 0.0) * 1024.0 * write_scale}
-    pair = [k for k in res["kernels"] if "gd_pair_kernel" in k]
+    pair = [k for k in res["kernels"] if "gd_pair_kernel<" in k]
     if pair:
-        res["hbm_bytes_per_launch"] = res["kernels"][pair[0]]["hbm_bytes"]
+        # Two-stream steps launch the pair kernel as two half-grids; the whole
+        # grid is the exchange points' swap passes and the one-stream launches.
+        # Per MC step: the two halves' bytes.
+        def by_grid(rows):
+            d = {}
+            for k, gs, v in rows:
+                if k == pair[0]:
+                    d.setdefault(gs, []).append(v)
+            return {gs: sum(v) / len(v) for gs, v in d.items()}
+        fg = by_grid(_counter_rows(fetch_dir, "FETCH_SIZE"))
+        wg = by_grid(_counter_rows(write_dir, "WRITE_SIZE"))
+        full = max(fg)
+        kib = lambda gs: fg[gs] * 1024.0 * fetch_scale + wg.get(gs, 0.0) * 1024.0 * write_scale
+        res["pair_by_grid"] = {str(gs): {"fetch_kib": fg[gs], "write_kib": wg.get(gs, 0.0),
+                                         "hbm_bytes": kib(gs)} for gs in sorted(fg)}
+        halves = [gs for gs in fg if gs != full]
+        res["hbm_bytes_whole_launch"] = kib(full)
+        if len(halves) == 1:
+            res["hbm_bytes_per_launch"] = 2.0 * kib(halves[0])
+        else:
+            res["hbm_bytes_per_launch"] = sum(kib(gs) for gs in halves) if halves else kib(full)
+        res["basis"] = ("one launch = one MC step's pair work: the sum of the two half-grid dispatches"
+                        if halves else "one whole-grid dispatch")
         res["kernel"] = pair[0]
     json.dump(res, open(out_json, "w"), indent=1)
     return res
