@@ -30,12 +30,15 @@ struct gd_job {
   GdItem *d_items = nullptr;
   GdBlock *d_blocks = nullptr;
   int n_items = 0, n_blocks = 0;
-  // the plain path's two stepping halves: blocks [0, nb_half[0]) and the rest,
-  // and their items (the job's items of those blocks, in the job's order)
-  GdItem *d_items_half[2] = {nullptr, nullptr};
-  int n_items_half[2] = {0, 0}, nb_half[2] = {0, 0};
-  int n_streams = 2;         // streams the plain path steps on (1 or 2)
-  hipStream_t stream2 = nullptr;
+  // the plain path's stepping parts: n_parts contiguous ranges of the blocks,
+  // [b0_part[k], b0_part[k] + nb_part[k]), each with its items (the job's items
+  // of those blocks, in the job's order) and a stream (part 0: the job's own)
+  static constexpr int kMaxParts = 4;
+  GdItem *d_items_part[kMaxParts] = {};
+  int n_items_part[kMaxParts] = {}, nb_part[kMaxParts] = {}, b0_part[kMaxParts] = {};
+  int n_parts = 0;
+  int n_streams = 2;         // streams the plain path steps on (1 .. kMaxParts)
+  hipStream_t part_stream[kMaxParts] = {};
   // protein points in source order, laid out on the device per tile count
   std::vector<float4> src_pco;     // [n_prt][P]
   std::vector<float4> src_pattr;   // [P]

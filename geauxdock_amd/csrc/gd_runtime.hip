@@ -101,10 +101,10 @@ gd_status KTimer::collect() {
   return GD_OK;
 }
 
-// Pair-phase timer of the two-stream plain path: per sampled MC step, the
-// span from the first start to the last end of its two half-launches (events
-// on both streams), every event timed from one reference recorded earlier on
-// the first stream (so no elapsed time is negative).
+// Pair-phase timer of the multi-stream plain path: per sampled MC step, the
+// span from the first start to the last end of its parts' launches (events on
+// every part's stream), every event timed from one reference recorded earlier
+// on the first stream (so no elapsed time is negative).
 struct SpanTimer {
   std::vector<hipEvent_t> ev;
   int used = 0;
@@ -129,27 +129,32 @@ struct SpanTimer {
     const hipError_t e = hipEventRecord(ref, s);
     return e == hipSuccess ? GD_OK : hip_fail(e, "hipEventRecord");
   }
-  gd_status record(hipStream_t s) {  // four per sample: A start, A end, B start, B end
+  gd_status record(hipStream_t s) {  // per sample: start and end of every part's launch
     hipEvent_t e = next();
     if (!e) return hip_fail(hipErrorOutOfMemory, "hipEventCreate");
     const hipError_t r = hipEventRecord(e, s);
     return r == hipSuccess ? GD_OK : hip_fail(r, "hipEventRecord");
   }
-  double halves_ms = 0.0, first_a = -1.0, last_a = -1.0;
-  int n_halves = 0;
-  gd_status collect() {  // after both streams are synchronised
-    for (int i = 0; i + 3 < used; i += 4) {
-      float t[4];
-      for (int k = 0; k < 4; ++k) {
-        const hipError_t e = hipEventElapsedTime(&t[k], ref, ev[i + k]);
+  double parts_ms = 0.0, first_a = -1.0, last_a = -1.0;
+  int n_part_launches = 0;
+  gd_status collect(int np) {  // after every part's stream is synchronised
+    for (int i = 0; i + 2 * np - 1 < used; i += 2 * np) {
+      float t0 = 0.0f, t_lo = 0.0f, t_hi = 0.0f;
+      for (int k = 0; k < np; ++k) {
+        float a, b;
+        hipError_t e = hipEventElapsedTime(&a, ref, ev[i + 2 * k]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&b, ref, ev[i + 2 * k + 1]);
         if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+        if (k == 0) t0 = t_lo = a, t_hi = b;
+        t_lo = std::min(t_lo, a);
+        t_hi = std::max(t_hi, b);
+        parts_ms += b - a;
+        ++n_part_launches;
       }
-      ms += std::max(t[1], t[3]) - std::min(t[0], t[2]);
+      ms += t_hi - t_lo;
       ++n;
-      halves_ms += (t[1] - t[0]) + (t[3] - t[2]);
-      n_halves += 2;
-      if (first_a < 0.0) first_a = t[0];
-      last_a = t[0];
+      if (first_a < 0.0) first_a = t0;
+      last_a = t0;
     }
     used = 0;
     return GD_OK;
@@ -775,40 +780,43 @@ gd_status sync_job_struct(gd_job *j) {
   for (int rb = 0; rb < j->h.Rp; rb += 64) blocks.push_back({0, rb});
   j->n_blocks = (int)blocks.size();
   j->n_items = (int)j->items.size();
-  // the two stepping halves of the plain path: the first half of the blocks
-  // and the rest, each with the job's items of its blocks (same order)
-  j->nb_half[0] = j->n_blocks / 2;
-  j->nb_half[1] = j->n_blocks - j->nb_half[0];
-  std::vector<GdItem> half_items;
-  for (int h = 0; h < 2; ++h) {
-    const int lo = h ? j->nb_half[0] * 64 : 0, hi = h ? j->h.Rp : j->nb_half[0] * 64;
-    j->n_items_half[h] = 0;
+  // the plain path's stepping parts: n_streams contiguous ranges of the
+  // blocks, as even as the block count allows, each with the job's items of
+  // its blocks (in the job's order)
+  const int np = std::max(1, std::min(j->n_streams, j->n_blocks));
+  j->n_parts = np;
+  std::vector<GdItem> part_items;
+  for (int k = 0; k < np; ++k) {
+    j->b0_part[k] = (int)((long)j->n_blocks * k / np);
+    j->nb_part[k] = (int)((long)j->n_blocks * (k + 1) / np) - j->b0_part[k];
+    const int lo = j->b0_part[k] * 64, hi = (j->b0_part[k] + j->nb_part[k]) * 64;
+    j->n_items_part[k] = 0;
     for (const GdItem &it : j->items)
       if (it.rb >= lo && it.rb < hi) {
-        half_items.push_back(it);
-        ++j->n_items_half[h];
+        part_items.push_back(it);
+        ++j->n_items_part[k];
       }
   }
   if (j->d_lists) hipFree(j->d_lists);
   j->d_lists = nullptr;
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t bi = (size_t)j->n_items * sizeof(GdItem), bb = (size_t)j->n_blocks * sizeof(GdBlock),
-               bh = half_items.size() * sizeof(GdItem);
+               bh = part_items.size() * sizeof(GdItem);
   j->lists_bytes = 0;
   e = hipMalloc(&j->d_lists, up(bi) + up(bb) + up(bh) + 256);
   if (e != hipSuccess) return hip_fail(e, "hipMalloc lists");
   j->lists_bytes = up(bi) + up(bb) + up(bh) + 256;
   j->d_items = (GdItem *)j->d_lists;
   j->d_blocks = (GdBlock *)((char *)j->d_lists + up(bi));
-  j->d_items_half[0] = (GdItem *)((char *)j->d_lists + up(bi) + up(bb));
-  j->d_items_half[1] = j->d_items_half[0] + j->n_items_half[0];
+  j->d_items_part[0] = (GdItem *)((char *)j->d_lists + up(bi) + up(bb));
+  for (int k = 1; k < np; ++k) j->d_items_part[k] = j->d_items_part[k - 1] + j->n_items_part[k - 1];
   if ((e = hipMemcpyAsync(j->d_items, j->items.data(), bi, hipMemcpyHostToDevice, j->stream)) != hipSuccess)
     return hip_fail(e, "hipMemcpyAsync items");
   if ((e = hipMemcpyAsync(j->d_blocks, blocks.data(), bb, hipMemcpyHostToDevice, j->stream)) != hipSuccess)
     return hip_fail(e, "hipMemcpyAsync blocks");
-  if (bh && (e = hipMemcpyAsync(j->d_items_half[0], half_items.data(), bh, hipMemcpyHostToDevice,
+  if (bh && (e = hipMemcpyAsync(j->d_items_part[0], part_items.data(), bh, hipMemcpyHostToDevice,
                                 j->stream)) != hipSuccess)
-    return hip_fail(e, "hipMemcpyAsync half items");
+    return hip_fail(e, "hipMemcpyAsync part items");
   e = hipStreamSynchronize(j->stream);
   return e == hipSuccess ? GD_OK : hip_fail(e, "hipStreamSynchronize");
 }
@@ -1037,7 +1045,8 @@ gd_status gd_job_destroy(gd_job *j) {
   if (j->d_rec) hipFree(j->d_rec);
   if (j->h_rec) hipHostFree(j->h_rec);
   if (j->copy_stream) hipStreamDestroy(j->copy_stream);
-  if (j->stream2) hipStreamDestroy(j->stream2);
+  for (hipStream_t ps : j->part_stream)
+    if (ps) hipStreamDestroy(ps);
   if (j->stream) hipStreamDestroy(j->stream);
   delete j;
   return GD_OK;
@@ -1071,9 +1080,12 @@ gd_status gd_tiling_get(const gd_job *j, gd_tiling *t) {
 }
 
 gd_status gd_job_set_streams(gd_job *j, int n_streams) {
-  if (!j || n_streams < 1 || n_streams > 2) return GD_ERR_ARG;
+  if (!j || n_streams < 1 || n_streams > gd_job::kMaxParts) return GD_ERR_ARG;
+  if (n_streams == j->n_streams) return GD_OK;
+  const gd_status st = ensure_device();
+  if (st != GD_OK) return st;
   j->n_streams = n_streams;
-  return GD_OK;
+  return sync_job_struct(j);  // the parts' item lists
 }
 
 gd_status gd_tiling_set(gd_job *j, const gd_tiling *t) {
@@ -1292,42 +1304,50 @@ static gd_status dock_impl(gd_job *j, const gd_mc_param *mc, const RecSink *sink
   KTimer tp;
   SpanTimer tspan;
   const bool timing = stats != nullptr;
-  // Two streams, each stepping half of the replica blocks: one half's reduce
-  // and finish kernels and its launch tail overlap the other half's pair
-  // work.  Between exchange points the halves share nothing; they join for
-  // every exchange point (which spans the whole job) and at dump ends.
-  const bool two = j->n_streams >= 2 && j->nb_half[0] > 0 && j->nb_half[1] > 0;
-  hipStream_t s2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  if (two) {
-    if (!j->stream2 && (e = hipStreamCreateWithFlags(&j->stream2, hipStreamNonBlocking)) != hipSuccess) {
-      if (d_xt) hipFree(d_xt);
-      return hip_fail(e, "hipStreamCreate");
+  // Several streams, each stepping one contiguous part of the replica blocks:
+  // one part's reduce and finish kernels and its launch tail overlap the other
+  // parts' pair work.  Between exchange points the parts share nothing; they
+  // join for every exchange point (which spans the whole job) and at dump ends.
+  const int np = j->n_parts;
+  bool multi = np >= 2;
+  for (int k = 0; k < np; ++k) multi = multi && j->nb_part[k] > 0;
+  hipStream_t ps[gd_job::kMaxParts] = {s};
+  hipEvent_t ev_fork = nullptr, ev_join[gd_job::kMaxParts] = {};
+  struct ForkGuard {
+    hipEvent_t *f, *jn;
+    ~ForkGuard() {
+      if (*f) hipEventDestroy(*f);
+      for (int k = 0; k < gd_job::kMaxParts; ++k)
+        if (jn[k]) hipEventDestroy(jn[k]);
     }
-    s2 = j->stream2;
-    if ((e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&ev_join, hipEventDisableTiming)) != hipSuccess) {
-      if (ev_fork) hipEventDestroy(ev_fork);
+  } fork_guard{&ev_fork, ev_join};
+  if (multi) {
+    e = hipSuccess;
+    for (int k = 1; k < np && e == hipSuccess; ++k) {
+      if (!j->part_stream[k]) e = hipStreamCreateWithFlags(&j->part_stream[k], hipStreamNonBlocking);
+      ps[k] = j->part_stream[k];
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming);
+    for (int k = 1; k < np && e == hipSuccess; ++k)
+      e = hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming);
+    if (e != hipSuccess) {
       if (d_xt) hipFree(d_xt);
-      return hip_fail(e, "hipEventCreate");
+      return hip_fail(e, "part streams");
     }
   }
-  struct ForkGuard {
-    hipEvent_t a, b;
-    ~ForkGuard() {
-      if (a) hipEventDestroy(a);
-      if (b) hipEventDestroy(b);
-    }
-  } fork_guard{ev_fork, ev_join};
-  // s2 continues after everything queued on s so far / s after everything on s2
+  // the other parts' streams continue after everything queued on s so far /
+  // s continues after everything queued on the other parts' streams
   const auto fork = [&]() {
     hipError_t r = hipEventRecord(ev_fork, s);
-    if (r == hipSuccess) r = hipStreamWaitEvent(s2, ev_fork, 0);
+    for (int k = 1; k < np && r == hipSuccess; ++k) r = hipStreamWaitEvent(ps[k], ev_fork, 0);
     return r == hipSuccess ? GD_OK : hip_fail(r, "stream fork");
   };
   const auto join = [&]() {
-    hipError_t r = hipEventRecord(ev_join, s2);
-    if (r == hipSuccess) r = hipStreamWaitEvent(s, ev_join, 0);
+    hipError_t r = hipSuccess;
+    for (int k = 1; k < np && r == hipSuccess; ++k) {
+      r = hipEventRecord(ev_join[k], ps[k]);
+      if (r == hipSuccess) r = hipStreamWaitEvent(s, ev_join[k], 0);
+    }
     return r == hipSuccess ? GD_OK : hip_fail(r, "stream join");
   };
   const auto t0 = std::chrono::steady_clock::now();
@@ -1335,7 +1355,7 @@ static gd_status dock_impl(gd_job *j, const gd_mc_param *mc, const RecSink *sink
   if (st == GD_OK) st = launch_transform(j->d_job, j->d_blocks, j->n_blocks, s);
   if (st == GD_OK) st = launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s);
   if (st == GD_OK) st = launch_finish(j->d_job, j->d_blocks, j->n_blocks, GD_MODE_INIT, 0, s);
-  if (st == GD_OK && two && timing) st = tspan.mark_ref(s);
+  if (st == GD_OK && multi && timing) st = tspan.mark_ref(s);
   int pending = -1;  // streamed dump whose copy is in flight
   for (int s1 = 0, k = 0; s1 < mc->steps_total && st == GD_OK; s1 += spd, ++k) {
     const int b = k & 1;
@@ -1346,21 +1366,19 @@ static gd_status dock_impl(gd_job *j, const gd_mc_param *mc, const RecSink *sink
         break;
       }
     }
-    if (two && (st = fork()) != GD_OK) break;
+    if (multi && (st = fork()) != GD_OK) break;
     const int s_end = std::min(mc->steps_total, s1 + spd);
     for (int step = s1; step < s_end; ++step) {
       const bool timed = timing && step % kTimedEvery == 0;
-      if (two) {
-        if (timed && (st = tspan.record(s)) != GD_OK) break;
-        if ((st = launch_pair(g.lc, j->d_job, j->d_items_half[0], j->n_items_half[0], s)) != GD_OK) break;
-        if (timed && (st = tspan.record(s)) != GD_OK) break;
-        if (timed && (st = tspan.record(s2)) != GD_OK) break;
-        if ((st = launch_pair(g.lc, j->d_job, j->d_items_half[1], j->n_items_half[1], s2)) != GD_OK) break;
-        if (timed && (st = tspan.record(s2)) != GD_OK) break;
-        if ((st = launch_finish(j->d_job, j->d_blocks, j->nb_half[0], GD_MODE_STEP, step, s)) != GD_OK ||
-            (st = launch_finish(j->d_job, j->d_blocks + j->nb_half[0], j->nb_half[1], GD_MODE_STEP, step,
-                                s2)) != GD_OK)
-          break;
+      if (multi) {
+        for (int k = 0; k < np && st == GD_OK; ++k) {
+          if (timed && (st = tspan.record(ps[k])) != GD_OK) break;
+          if ((st = launch_pair(g.lc, j->d_job, j->d_items_part[k], j->n_items_part[k], ps[k])) != GD_OK) break;
+          if (timed) st = tspan.record(ps[k]);
+        }
+        for (int k = 0; k < np && st == GD_OK; ++k)
+          st = launch_finish(j->d_job, j->d_blocks + j->b0_part[k], j->nb_part[k], GD_MODE_STEP, step, ps[k]);
+        if (st != GD_OK) break;
       } else {
         if (timed && (st = tp.begin(s)) != GD_OK) break;
         if ((st = launch_pair(g.lc, j->d_job, j->d_items, j->n_items, s)) != GD_OK) break;
@@ -1369,7 +1387,7 @@ static gd_status dock_impl(gd_job *j, const gd_mc_param *mc, const RecSink *sink
       }
       if (n_rounds > 0 && (step + 1) % spe == 0 && (step + 1) / spe <= n_rounds) {
         const int r = (step + 1) / spe - 1;
-        if (two && (st = join()) != GD_OK) break;
+        if (multi && (st = join()) != GD_OK) break;
         if (tx && (st = launch_exchange(j->d_job, d_ex + r, 1, s)) != GD_OK) break;
         for (int axis = 0; axis < 2; ++axis)
           if ((axis ? px : lx) &&
@@ -1377,11 +1395,11 @@ static gd_status dock_impl(gd_job *j, const gd_mc_param *mc, const RecSink *sink
                                 nullptr, pair, s)) != GD_OK)
             break;
         if (st != GD_OK) break;
-        if (two && (st = fork()) != GD_OK) break;
+        if (multi && (st = fork()) != GD_OK) break;
       }
     }
     if (st != GD_OK) break;
-    if (two && (st = join()) != GD_OK) break;
+    if (multi && (st = join()) != GD_OK) break;
     j->last_half = record ? b : -1;
     if (!sink) continue;
     const float *d_half = g.rec + (size_t)b * half_floats;
@@ -1419,8 +1437,8 @@ static gd_status dock_impl(gd_job *j, const gd_mc_param *mc, const RecSink *sink
   {
     hipError_t e2 = hipStreamSynchronize(s);  // also covers steps_total == 0
     if (st == GD_OK && e2 != hipSuccess) st = hip_fail(e2, "hipStreamSynchronize");
-    if (s2) {
-      e2 = hipStreamSynchronize(s2);
+    for (int k = 1; multi && k < np; ++k) {
+      e2 = hipStreamSynchronize(ps[k]);
       if (st == GD_OK && e2 != hipSuccess) st = hip_fail(e2, "hipStreamSynchronize");
     }
     if (j->copy_stream) {
@@ -1429,23 +1447,23 @@ static gd_status dock_impl(gd_job *j, const gd_mc_param *mc, const RecSink *sink
     }
   }
   if (st == GD_OK && sink && !sink->blocking && pending >= 0) deliver(pending, pending & 1);
-  if (st == GD_OK && timing) st = two ? tspan.collect() : tp.collect();
+  if (st == GD_OK && timing) st = multi ? tspan.collect(np) : tp.collect();
   const auto t1 = std::chrono::steady_clock::now();
   if (d_xt) hipFree(d_xt);
   if (st != GD_OK) return st;
   if (stats) {
     stats->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    stats->ms_pair_kernel = two ? tspan.halves_ms : tp.ms;
+    stats->ms_pair_kernel = multi ? tspan.parts_ms : tp.ms;
     stats->ms_finish_kernel = 0.0;
-    stats->n_pair_launch = two ? tspan.n_halves : tp.n;
+    stats->n_pair_launch = multi ? tspan.n_part_launches : tp.n;
     stats->pair_evals = (double)g.R * g.L * g.P * (mc->steps_total + 1);
     stats->replica_steps = (double)g.R * mc->steps_total;
     // timed steps are 0, kTimedEvery, ...: first to last start over the steps between
-    const int n_samples = two ? tspan.n : tp.n;
-    const double span_first_last = two ? tspan.last_a - tspan.first_a : tp.first_to_last_ms;
+    const int n_samples = multi ? tspan.n : tp.n;
+    const double span_first_last = multi ? tspan.last_a - tspan.first_a : tp.first_to_last_ms;
     stats->ms_step_period =
         n_samples >= 2 ? span_first_last / ((double)(n_samples - 1) * kTimedEvery) : 0.0;
-    stats->ms_pair_span = two ? (tspan.n ? tspan.ms / tspan.n : 0.0) : (tp.n ? tp.ms / tp.n : 0.0);
+    stats->ms_pair_span = multi ? (tspan.n ? tspan.ms / tspan.n : 0.0) : (tp.n ? tp.ms / tp.n : 0.0);
   }
   return read_out(j, out);
 }

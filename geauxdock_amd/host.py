@@ -289,7 +289,7 @@ class DeviceJob:
         return t.atoms_per_lane, t.tile_points
 
     def set_streams(self, n):
-        """Streams gd_dock steps this job on (1 or 2)."""
+        """Streams gd_dock steps this job on (1 to 4)."""
         check(lib().gd_job_set_streams(self.h, n), "gd_job_set_streams")
 
     def set_tiling(self, atoms_per_lane, tile_points):

@@ -269,10 +269,11 @@ typedef struct {
   int waves_per_cu;         /* persistent grid size per CU */
 } gd_tiling;
 gd_status gd_tiling_get(const gd_job *job, gd_tiling *t);
-/* Streams gd_dock steps the job on (1 or 2, default 2): with two, each steps
- * half of the replica blocks between exchange points, so one half's reduce
- * and finish kernels and launch tail overlap the other half's pair work.  The
- * results are the same bits either way. */
+/* Streams gd_dock steps the job on (1 to 4, default 2): with n, each steps one
+ * of n contiguous parts of the replica blocks between exchange points, so one
+ * part's reduce and finish kernels and launch tail overlap the other parts'
+ * pair work.  The results are the same bits either way.  Waits for the job's
+ * stream (the parts' work lists are rebuilt). */
 gd_status gd_job_set_streams(gd_job *job, int n_streams);
 gd_status gd_tiling_set(gd_job *job, const gd_tiling *t);
 /* Sweep atoms per lane x protein tile counts on `job` for `steps` MC steps

@@ -243,29 +243,42 @@ def test_record_csv_sink(small_job, tmp_path):
     assert lines[1:] == want
 
 
-def test_two_streams_same_bits(ene):
-    """Stepping the two halves of the replica blocks on two streams gives the
-    same bits as one stream: exchange on every axis, records streamed, an odd
-    block count (7 blocks: halves of 3 and 4)."""
+def test_stream_parts_same_bits(ene):
+    """Stepping contiguous parts of the replica blocks on 2, 3 and 4 streams
+    gives the same bits as one stream: exchange on every axis, records
+    streamed, an odd block count (7 blocks: parts of 3+4, 2+2+3, 1+2+2+2).
+    One job object switches between stream counts, so the parts' work lists
+    are rebuilt in place."""
     job = g.synth_complex(900, 20, 14, 4, 8, seed=23, ene=ene)   # 448 replicas: 7 blocks
     mc = g.make_mc(90, steps_per_dump=40, steps_per_exchange=15, move_mode="gaussian", seed=13,
                    axes=("temp", "lig", "prt"))
     res = []
-    for n in (1, 2):
-        dev = g.DeviceJob(job)
+    dev = g.DeviceJob(job)
+    for n in (1, 2, 3, 4, 2):
         dev.set_streams(n)
         dumps = []
         out, st = dev.dock_stream(mc, lambda k, s0, rec, cnt: dumps.append((rec, cnt)), stats=True)
-        assert st.n_pair_launch > 0 and st.ms_pair_kernel > 0
+        assert st.n_pair_launch > 0 and st.ms_pair_kernel > 0 and st.ms_step_period > 0
         res.append((out, dumps))
-    (a, da), (b, db) = res
-    for f in ("energy", "movematrix", "idx_tmp", "n_accept", "n_exchange", "n_swap_lig", "n_swap_prt",
-              "best_energy"):
-        assert np.array_equal(getattr(a, f), getattr(b, f)), f
-    for (ra, ca), (rb, cb) in zip(da, db):
-        assert np.array_equal(ca, cb)
-        for r in range(job.n_rep):
-            assert np.array_equal(ra[r, :ca[r]], rb[r, :cb[r]])
+    dev.close()
+    a, da = res[0]
+    for b, db in res[1:]:
+        for f in ("energy", "movematrix", "idx_tmp", "n_accept", "n_exchange", "n_swap_lig", "n_swap_prt",
+                  "best_energy"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
+        assert len(da) == len(db)
+        for (ra, ca), (rb, cb) in zip(da, db):
+            assert np.array_equal(ca, cb)
+            for r in range(job.n_rep):
+                assert np.array_equal(ra[r, :ca[r]], rb[r, :cb[r]])
+
+
+def test_stream_count_bounds(small_job):
+    dev = g.DeviceJob(small_job)
+    for bad in (0, 5):
+        with pytest.raises(g.GDError):
+            dev.set_streams(bad)
+    dev.close()
 
 
 def test_dock_is_deterministic(small_job):
