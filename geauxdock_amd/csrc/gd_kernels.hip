@@ -220,7 +220,9 @@ __device__ __forceinline__ void gd_pair_one(
         gd_hdb_axis(lyj, c.y, r2d);
         gd_hdb_axis(lxj, c.x, r2d);
         const double r0 = gd_f2d(rr);
-        const double rd = __builtin_fma(__builtin_fma(-r0, r0, r2d), gd_f2d(0.5f * ri), r0);
+        // 1/(2 r) from rr, not from ri: ri is then dead once g is chosen, and
+        // g shares its register (no copy on the far path)
+        const double rd = __builtin_fma(__builtin_fma(-r0, r0, r2d), gd_f2d(0.5f * gd_rcp(rr)), r0);
         const double hd = (rd - gd_f2d_s(hdb0)) * gd_f2d_s(hdb1);
         const double q = (hd * hd) * GD_HALF_LOG2E_D;
         const float fq = (float)__builtin_amdgcn_fract(q);

@@ -274,11 +274,21 @@ def test_stream_parts_same_bits(ene):
 
 
 def test_stream_count_bounds(small_job):
+    """Stream counts outside 1-4 are refused; more streams than replica
+    blocks step the blocks there are (here one), with the same bits."""
     dev = g.DeviceJob(small_job)
     for bad in (0, 5):
         with pytest.raises(g.GDError):
             dev.set_streams(bad)
+    mc = g.make_mc(30, steps_per_exchange=10, move_mode="gaussian", seed=4)
+    outs = []
+    for n in (1, 4):
+        dev.set_streams(n)
+        outs.append(dev.dock(mc)[0])
     dev.close()
+    assert -(-small_job.n_rep // 64) < 4
+    for f in ("energy", "movematrix", "n_accept"):
+        assert np.array_equal(getattr(outs[0], f), getattr(outs[1], f)), f
 
 
 def test_dock_is_deterministic(small_job):
