@@ -408,8 +408,9 @@ gd_mc_param tune_mc(int steps) {
 }  // namespace
 
 // One job per launch: atoms per lane {4, 8} x protein tile counts, each run
-// for `steps` MC steps; the fastest mean pair launch (HIP events) wins and is
-// applied to this job (and only to it).
+// for `steps` MC steps on the job's streams; the shortest steady-state MC
+// step period (HIP events at every 16th step: 33 steps or more; fewer fall
+// back to wall time per step) wins and is applied to this job (only to it).
 extern "C" gd_status gd_autotune(gd_job *j, int steps, gd_tiling *best, double *best_ms) {
   if (!j || steps <= 0) return GD_ERR_ARG;
   gd_status st = ensure_device();
@@ -418,7 +419,7 @@ extern "C" gd_status gd_autotune(gd_job *j, int steps, gd_tiling *best, double *
   const int P = g0.P;
   std::vector<std::pair<int, int>> cand;  // (lc, ptile)
   for (int lc : {4, 8})
-    for (int nP : {40, 48, 56, 64, 72, 80, 88, 96, 112, 128, 160}) {
+    for (int nP : {16, 24, 32, 40, 48, 56, 64, 72, 80, 88, 96, 112, 128, 160}) {
       const int pt = (P + nP - 1) / nP;
       if (pt >= 4 && std::find(cand.begin(), cand.end(), std::make_pair(lc, pt)) == cand.end())
         cand.push_back({lc, pt});
@@ -437,7 +438,9 @@ extern "C" gd_status gd_autotune(gd_job *j, int steps, gd_tiling *best, double *
     if ((st = sync_job_struct(j)) != GD_OK) return st;
     gd_stats s;
     if ((st = gd_dock(j, &mc, nullptr, &s)) != GD_OK) return st;
-    const double t = s.n_pair_launch ? s.ms_pair_kernel / s.n_pair_launch : 1e300;
+    // what the job pays per MC step: the steady-state step period on the
+    // job's streams (on several streams a pair launch is only part of a step)
+    const double t = s.ms_step_period > 0.0 ? s.ms_step_period : s.ms_total / steps;
     if (t < best_t) {
       best_t = t;
       bestc = c;

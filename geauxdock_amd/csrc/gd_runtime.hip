@@ -263,9 +263,18 @@ int lg2_floor(int x) {
   return b;
 }
 
+// Half-octave index: 2 floor(log2 x), plus 1 in the octave's upper part
+// [1.5 2^k, 2^(k+1)).  One-job tilings follow the ligand's chunk count closely
+// (the best tile count keeps the items of a launch near the machine's group
+// slots), so their table is keyed on half octaves of L.
+int hlg2_floor(int x) {
+  const int k = lg2_floor(std::max(1, x));
+  return 2 * k + (k > 0 && 2 * x >= 3 << k ? 1 : 0);
+}
+
 static const GdTuned *tuned_entry(int mode, int L, int P, int R) {
-  const int lgL = lg2_floor(std::max(1, L)), lgP = lg2_floor(std::max(1, P)),
-            lgR = lg2_floor(std::max(1, R));
+  const int lgL = mode == GD_TUNE_PLAIN ? hlg2_floor(L) : lg2_floor(std::max(1, L)),
+            lgP = lg2_floor(std::max(1, P)), lgR = lg2_floor(std::max(1, R));
   for (const GdTuned &t : kTunedTable) {
     if (t.mode != mode) continue;
     if (mode == GD_TUNE_WAVES ||

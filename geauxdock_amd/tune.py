@@ -4,7 +4,8 @@
 
 Then rebuild (python -m geauxdock_amd.build): the library looks a job's
 tiling up in that table by size bucket (floor(log2) of ligand atoms L,
-protein points P and replicas R), so a tiling is a function of the job alone.
+protein points P and replicas R; half octaves of L for one-job tilings), so a
+tiling is a function of the job alone.
 
   shared launches (batches, serving): for each (L, P) bucket of the screening
       sizes (L 12-80, P 1,500-9,000), gd_autotune_shared on a batch of 16
@@ -12,8 +13,10 @@ protein points P and replicas R), so a tiling is a function of the job alone.
       replicas), 32 MC steps per candidate: protein tile points x persistent
       waves per CU.  The waves per CU most buckets pick is kept for all.
   one job per launch: for each (L, P, R) bucket of the configurations' plain
-      jobs (R = 25, 220, 440, 880, 1,760), gd_autotune on one synthetic
-      complex, 64 MC steps per candidate: atoms per lane x tile count.
+      jobs (R = 25, 220, 440, 880, 1,760), gd_autotune on two synthetic
+      complexes of the bucket's L range on the library's default streams, 128
+      MC steps per candidate: atoms per lane x tile count, ranked by MC step
+      period; differing winners are settled on both complexes.
 
 The sweep candidates are the C functions' own (geauxdock_amd/csrc/gd_batch.hip).
 """
@@ -32,6 +35,8 @@ import geauxdock_amd as g  # noqa: E402
 from geauxdock_amd import abi  # noqa: E402
 
 L_REP = {3: 14, 4: 24, 5: 46, 6: 72}          # representative L of each log2 bucket (12-80)
+# one-job tilings are keyed on half octaves of L (hlg2): two ligand sizes per bucket
+L_REPS = {7: (12, 15), 8: (16, 22), 9: (25, 31), 10: (34, 44), 11: (50, 60), 12: (66, 78)}
 P_REP = {10: 1700, 11: 3000, 12: 6000, 13: 9000}  # representative P of each log2 bucket (1,500-9,000)
 R_SHAPES = [(5, 5, 1), (5, 11, 4), (10, 11, 4), (20, 11, 4), (20, 11, 8)]   # 25 .. 1,760 replicas
 
@@ -41,6 +46,13 @@ def lg2(x):
     while (2 << b) <= x:
         b += 1
     return b
+
+
+def hlg2(x):
+    """Half-octave index: 2 floor(log2 x), plus 1 from 1.5 2^k on (the
+    library's key for one-job tilings)."""
+    k = lg2(max(1, x))
+    return 2 * k + (1 if k > 0 and 2 * x >= (3 << k) else 0)
 
 
 def tune_shared(ene, steps=32, n_jobs=16):
@@ -67,30 +79,58 @@ def tune_shared(ene, steps=32, n_jobs=16):
     return entries
 
 
-def tune_plain(ene, steps=64):
+def _period(dev, lc, pt, steps):
+    """MC step period (ms) of a device job at one tiling."""
+    dev.set_tiling(lc, pt)
+    _, st = dev.dock(g.make_mc(steps, move_mode="gaussian", seed=1), stats=True)
+    return st.ms_step_period
+
+
+def tune_plain(ene, steps=128):
+    """Each (L, P, R) bucket on two ligand sizes of its L range (same P and R):
+    gd_autotune on each; where the winners differ, every winner is timed on
+    both jobs and the one with the smallest summed period relative to each
+    job's own best is kept."""
     entries = []
-    for lgL, L in L_REP.items():
+    for hL, Ls in L_REPS.items():
+        assert all(hlg2(L) == hL for L in Ls), Ls
         for lgP, P in P_REP.items():
             for nl, np_, nt in R_SHAPES:
-                h = g.synth_complex(P, L, nl, np_, nt, seed=31 * lgL + lgP, ene=ene)
-                d = g.DeviceJob(h)
-                t = abi.gd_tiling()
-                ms = C.c_double()
-                g.check(g.lib().gd_autotune(d.h, steps, C.byref(t), C.byref(ms)), "gd_autotune")
-                d.close()
-                n_tiles = -(-P // t.tile_points)
-                entries.append(("GD_TUNE_PLAIN", lgL, lgP, lg2(h.n_rep), t.atoms_per_lane, n_tiles))
-                print("plain L=%d P=%d R=%d: %d atoms/lane, %d tiles of %d points (%.3f ms/launch)"
-                      % (L, P, h.n_rep, t.atoms_per_lane, n_tiles, t.tile_points, ms.value), flush=True)
+                devs, wins = [], []
+                for k, L in enumerate(Ls):
+                    h = g.synth_complex(P, L, nl, np_, nt, seed=31 * hL + lgP + 1000 * k, ene=ene)
+                    d = g.DeviceJob(h)
+                    t = abi.gd_tiling()
+                    ms = C.c_double()
+                    g.check(g.lib().gd_autotune(d.h, steps, C.byref(t), C.byref(ms)), "gd_autotune")
+                    devs.append(d)
+                    wins.append(((t.atoms_per_lane, t.tile_points), ms.value))
+                cands = sorted({w[0] for w in wins})
+                if len(cands) == 1:
+                    best = cands[0]
+                else:
+                    score = {c: sum(_period(d, c[0], c[1], steps) / w[1] for d, w in zip(devs, wins))
+                             for c in cands}
+                    best = min(cands, key=lambda c: score[c])
+                for d in devs:
+                    d.close()
+                n_tiles = -(-P // best[1])
+                R = nl * np_ * nt
+                entries.append(("GD_TUNE_PLAIN", hL, lgP, lg2(R), best[0], n_tiles))
+                print("plain L=%s P=%d R=%d: %d atoms/lane, %d tiles of %d points (winners %s)"
+                      % ("/".join(map(str, Ls)), P, R, best[0], n_tiles, best[1],
+                         ", ".join("%d x %d: %.4f ms" % (w[0][0], w[0][1], w[1]) for w in wins)), flush=True)
     return entries
 
 
 HEADER = """// gd_tiling_table.h -- tiling table of the pair kernel, written by
 // `python -m geauxdock_amd.tune` from sweeps on an MI355X (gd_autotune and
 // gd_autotune_shared).  A job's tiling is looked up here by size bucket
-// (floor(log2) of its ligand atoms L, protein points P and replicas R), so it
-// is a function of the job alone, never of what else the process ran.
-//   GD_TUNE_PLAIN  (lgL, lgP, lgR) -> atoms per lane, protein tile count
+// (floor(log2) of its ligand atoms L, protein points P and replicas R; for
+// one-job tilings half octaves of L: hlgL = 2 floor(log2 L), plus 1 from
+// 1.5 2^floor(log2 L) on), so it is a function of the job alone, never of what
+// else the process ran.
+//   GD_TUNE_PLAIN  (hlgL, lgP, lgR) -> atoms per lane, protein tile count
 //   GD_TUNE_SHARED (lgL, lgP)      -> protein tile points (4 atoms per lane)
 //   GD_TUNE_WAVES  ()              -> persistent pair kernel waves per CU
 // Buckets without an entry use the built-in rule (gd_runtime.hip).
@@ -112,6 +152,16 @@ static const GdTuned kTunedTable[] = {
 """
 
 
+def read_table(path, modes):
+    """Entries of the given modes from an existing table header."""
+    import re
+    out = []
+    for m in re.finditer(r"\{(GD_TUNE_\w+), (-?\d+), (-?\d+), (-?\d+), (-?\d+), (-?\d+)\},", open(path).read()):
+        if m.group(1) in modes:
+            out.append((m.group(1),) + tuple(int(x) for x in m.groups()[1:]))
+    return out
+
+
 def write_table(entries, path, device):
     rows = ["  // %s" % device]
     rows += ["  {%s, %d, %d, %d, %d, %d}," % e for e in entries]
@@ -123,13 +173,21 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "csrc", "gd_tiling_table.h"))
     ap.add_argument("--shared-only", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="re-tune the one-job entries, keeping the shared-launch entries of the "
+                         "table the library was built with")
     args = ap.parse_args()
     g.check(g.lib().gd_init(0), "gd_init")
     name = C.create_string_buffer(128)
     ncu, clk, hbm = C.c_int(), C.c_int(), C.c_size_t()
     g.lib().gd_device_info(name, 128, C.byref(ncu), C.byref(clk), C.byref(hbm))
     ene = g.load_enepara()
-    entries = tune_shared(ene)
+    if args.plain_only:
+        src = os.path.join(HERE, "csrc", "gd_tiling_table.h")   # the table the library was built with
+        entries = read_table(src, ("GD_TUNE_SHARED", "GD_TUNE_WAVES"))
+        print("kept %d shared-launch entries of %s" % (len(entries), src), flush=True)
+    else:
+        entries = tune_shared(ene)
     if not args.shared_only:
         entries += tune_plain(ene)
     write_table(entries, args.out, "tuned on %s" % name.value.decode())

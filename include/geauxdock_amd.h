@@ -277,9 +277,10 @@ gd_status gd_tiling_get(const gd_job *job, gd_tiling *t);
 gd_status gd_job_set_streams(gd_job *job, int n_streams);
 gd_status gd_tiling_set(gd_job *job, const gd_tiling *t);
 /* Sweep atoms per lane x protein tile counts on `job` for `steps` MC steps
- * each and apply the fastest to this job only (the reference's counterpart,
- * the disabled CalcThreadBlockShape of backend_gpu/kernel_cuda_l2_util.cu:40-67,
- * is a formula). */
+ * each (33 or more for an event-timed period) on the job's streams, and apply
+ * the one with the shortest MC step period (*best_ms) to this job only (the
+ * reference's counterpart, the disabled CalcThreadBlockShape of
+ * backend_gpu/kernel_cuda_l2_util.cu:40-67, is a formula). */
 gd_status gd_autotune(gd_job *job, int steps, gd_tiling *best, double *best_ms);
 /* Sweep the tiling of shared launches (protein tile points x persistent-grid
  * waves per CU, 4 atoms per lane) on a batch of these jobs, `steps` MC steps
