@@ -11,12 +11,16 @@ pass per conformer axis).  One bench "step" = one exchange interval (100 MC
 steps of all 1,760 replicas + one exchange point), so the default --steps 100 is the whole
 10,000-step job.  FP32 arithmetic, synthetic data (seeded generator).
 
-Prints one JSON line (value = replica-steps/s).  The roofline object covers
-the dominant kernel (gd_pair_kernel<LC>, LC ligand atoms per lane): algorithmic FP32 flops per launch /
-its mean launch time from HIP events recorded on the job's stream around
-the launch of every 16th MC step (625 launches over the job; the library
-samples so that the records, about 4 us each, stay under 1% of the run).  cpu_baseline times the CPU oracle (a restatement of the
-reference CPU backend, in the reference's FP32 expression order) on a bounded sample of the same workload.
+Prints one JSON line (value = replica-steps/s, on the library's default two
+streams).  The roofline object covers the dominant kernel (gd_pair_kernel<LC>,
+LC ligand atoms per lane) on a pipeline basis: algorithmic FP32 flops of one
+MC step's pair work / the steady-state MC step period, from HIP events at the
+first stream's pair launch of every 16th MC step (the whole step, tail
+kernels and exchange points included, is charged to the pair kernel); the
+half-launch mean, the step's pair span and an isolated one-stream launch
+are reported beside it.  cpu_baseline times the CPU oracle (a restatement of
+the reference CPU backend, in the reference's FP32 expression order) on a
+bounded sample of the same workload.
 
 The same line carries, unless --no-extra:
   reference_shape  the headline job in the reference's plain shape (dumps of
@@ -26,9 +30,13 @@ The same line carries, unless --no-extra:
   c3               the batched screen in full (256 ligands x 440 replicas x
                    5,000 steps against one 4,000-point target) against the plain
                    path in both shapes on 32 randomly drawn jobs, with its own
-                   roofline and CPU baseline.
---workload c1|c3|c4|c5 print one of these alone (c4: heterogeneous batch,
-c5: Poisson serving at 50/200/800 arrivals/s).
+                   roofline and CPU baseline;
+  c1_batched       256 complexes of the CPU reference configuration's shape
+                   (distinct seeds; 25 replicas, 3,000 steps each) batched,
+                   against the plain path in both shapes on 16 of them: the
+                   small jobs one launch cannot fill the GPU with.
+--workload c1|c1b|c3|c4|c5 print one of these alone (c1b: c1_batched, c4:
+heterogeneous batch, c5: Poisson serving at 50/200/800 arrivals/s).
 """
 import argparse
 import ctypes as C
@@ -271,7 +279,9 @@ def _sizes(rng, n, kind):
     out = []
     for i in range(n):
         L = int(rng.integers(12, 81))
-        if kind == "c3":
+        if kind == "c1b":   # the CPU reference configuration's shape
+            out.append(dict(P=1200, L=24, nlig=5, nprt=5, ntmp=1))
+        elif kind == "c3":
             out.append(dict(P=4000, L=L, nlig=10, nprt=11, ntmp=4, prot=3))
         else:
             P = int(round(np.exp(rng.uniform(np.log(1500), np.log(9000)))))
@@ -284,7 +294,10 @@ def _hosts(kind, sizes, seed0):
     import geauxdock_amd as g
     hosts = []
     for i, z in enumerate(sizes):
-        if kind == "c5":
+        if kind == "c1b":
+            hosts.append(g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.044,
+                                         ceiling_temp=0.044, seed=seed0 + i))
+        elif kind == "c5":
             hosts.append(g.synth_complex(z["P"], z["L"], z["nlig"], z["nprt"], z["ntmp"], floor_temp=0.004,
                                          ceiling_temp=0.04, seed=seed0 + i))
         else:   # C3: one target for every ligand; C4: 32 ligands per target
@@ -310,8 +323,10 @@ def batched(kind, n_jobs, mc_steps, batch_size=0, plain_sample=0, roofline_steps
     rng = np.random.default_rng(3 if kind == "c3" else 4)
     sizes = _sizes(rng, n_jobs, kind)
     hosts = _hosts(kind, sizes, 100)
-    mcs = [g.make_mc(mc_steps, steps_per_exchange=100, move_mode="gaussian", seed=5, job_id=i,
-                     axes=AXES) for i in range(n_jobs)]
+    # the CPU reference configuration's shape has one temperature and no exchange
+    spe, axes = (0, ()) if kind == "c1b" else (100, AXES)
+    mcs = [g.make_mc(mc_steps, steps_per_exchange=spe, move_mode="gaussian", seed=5, job_id=i,
+                     axes=axes) for i in range(n_jobs)]
     mca = (abi.gd_mc_param * n_jobs)(*mcs)
     outs = [g.host.ReplicaOut(h.n_rep) for h in hosts]
     oa = (abi.gd_replica_out * n_jobs)(*[o.c for o in outs])
@@ -344,8 +359,8 @@ def batched(kind, n_jobs, mc_steps, batch_size=0, plain_sample=0, roofline_steps
     nb = min(n_jobs, bs)
     devs = [g.DeviceJob(h) for h in hosts[:nb]]
     arr = (C.c_void_p * nb)(*[d.h.value for d in devs])
-    rmc = (abi.gd_mc_param * nb)(*[g.make_mc(roofline_steps, steps_per_exchange=100, seed=5, job_id=i,
-                                             axes=AXES) for i in range(nb)])
+    rmc = (abi.gd_mc_param * nb)(*[g.make_mc(roofline_steps, steps_per_exchange=spe, seed=5, job_id=i,
+                                             axes=axes) for i in range(nb)])
     st = abi.gd_stats()
     g.check(g.lib().gd_batch_dock(arr, nb, rmc, None, C.byref(st)), "gd_batch_dock (timed)")
     flops = sum(algorithmic_flops_per_step(h)[0] for h in hosts[:nb])
@@ -361,8 +376,8 @@ def batched(kind, n_jobs, mc_steps, batch_size=0, plain_sample=0, roofline_steps
     tp = tr = 0.0
     for q, i in enumerate(which):
         tp += timed_plain(hosts[i], mcs[i], False)
-        mc_r = g.make_mc(mc_steps, steps_per_dump=200, steps_per_exchange=100, move_mode="gaussian",
-                         seed=5, job_id=i, axes=AXES)
+        mc_r = g.make_mc(mc_steps, steps_per_dump=200, steps_per_exchange=spe, move_mode="gaussian",
+                         seed=5, job_id=i, axes=axes)
         tr += timed_plain(hosts[i], mc_r, True)
         if (q + 1) % 16 == 0:
             print("[%s] plain legs: %d jobs in %.1f s" % (kind, q + 1, tp + tr), file=sys.stderr, flush=True)
@@ -614,7 +629,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100, help="exchange intervals (100 MC steps each)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "copy"], default="c2",
+    ap.add_argument("--workload", choices=["c1", "c1b", "c2", "c3", "c4", "c5", "copy"], default="c2",
                     help="c2 = the headline line; c1 the CPU reference configuration on CPU and GPU; "
                          "c3/c4 batched vs plain; c5 serving (extra lines); "
                          "copy = a plain 256 MiB device copy, the calibration for HBM counters")
@@ -655,7 +670,7 @@ def main():
     if args.workload == "c1":
         print(json.dumps(c1_extra()), flush=True)
         return
-    if args.workload in ("c3", "c4"):
+    if args.workload in ("c1b", "c3", "c4"):
         print(json.dumps(batched(args.workload, args.jobs, args.mc_steps, args.batch_size,
                                  args.plain_sample, cpu=not args.no_cpu)), flush=True)
         return
@@ -785,6 +800,8 @@ def main():
         line["c1"] = c1_extra()
         print("[bench] C1 done; C3 (256 ligands x 5000 steps, batched)", file=sys.stderr, flush=True)
         line["c3"] = batched("c3", 256, 5000, 0, 32, cpu=not args.no_cpu)
+        print("[bench] C3 done; 256 C1-shaped complexes batched", file=sys.stderr, flush=True)
+        line["c1_batched"] = batched("c1b", 256, 3000, 0, 16, cpu=False)
     print(json.dumps(line), flush=True)
 
 

@@ -57,3 +57,18 @@ def test_cpu_leg_describes_its_cores():
     f = bench.cpu_fields(bench.cpu_threads())
     assert 1 <= f["cores"] <= min(16, phys)
     assert f["cpu_model"] and f["cpu_model"] != "unknown"
+
+
+def test_batched_workload_shapes():
+    """The batched legs' complexes: C1's shape (P=1200 x 5, L=24 x 5, one
+    temperature: 25 replicas) for c1b, one 4,000-point target with 12-80-atom
+    ligands x 440 replicas for c3, logU[1500, 9000] proteins for c4."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    import bench
+    z = bench._sizes(np.random.default_rng(0), 4, "c1b")
+    assert all((d["P"], d["L"], d["nlig"] * d["nprt"] * d["ntmp"]) == (1200, 24, 25) for d in z)
+    z = bench._sizes(np.random.default_rng(0), 64, "c3")
+    assert all(d["P"] == 4000 and 12 <= d["L"] <= 80 and d["nlig"] * d["nprt"] * d["ntmp"] == 440 for d in z)
+    z = bench._sizes(np.random.default_rng(0), 64, "c4")
+    assert all(1500 <= d["P"] <= 9000 and d["nlig"] * d["nprt"] * d["ntmp"] == 220 for d in z)
