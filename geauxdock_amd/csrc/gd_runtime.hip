@@ -263,17 +263,14 @@ int lg2_floor(int x) {
   return b;
 }
 
-// Half-octave index: 2 floor(log2 x), plus 1 in the octave's upper part
-// [1.5 2^k, 2^(k+1)).  One-job tilings follow the ligand's chunk count closely
-// (the best tile count keeps the items of a launch near the machine's group
-// slots), so their table is keyed on half octaves of L.
-int hlg2_floor(int x) {
-  const int k = lg2_floor(std::max(1, x));
-  return 2 * k + (k > 0 && 2 * x >= 3 << k ? 1 : 0);
-}
+// One-job tilings are keyed on the ligand's 4-atom chunk count, ceil(L / 4):
+// on two streams the best tile count keeps each half-launch's items (chunks x
+// tile groups x replica blocks) near the machine's group slots, so it follows
+// the chunk count exactly rather than a size class of L.
+int chunk_key(int L) { return (std::max(1, L) + 3) / 4; }
 
 static const GdTuned *tuned_entry(int mode, int L, int P, int R) {
-  const int lgL = mode == GD_TUNE_PLAIN ? hlg2_floor(L) : lg2_floor(std::max(1, L)),
+  const int lgL = mode == GD_TUNE_PLAIN ? chunk_key(L) : lg2_floor(std::max(1, L)),
             lgP = lg2_floor(std::max(1, P)), lgR = lg2_floor(std::max(1, R));
   for (const GdTuned &t : kTunedTable) {
     if (t.mode != mode) continue;

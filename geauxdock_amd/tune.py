@@ -4,19 +4,20 @@
 
 Then rebuild (python -m geauxdock_amd.build): the library looks a job's
 tiling up in that table by size bucket (floor(log2) of ligand atoms L,
-protein points P and replicas R; half octaves of L for one-job tilings), so a
-tiling is a function of the job alone.
+protein points P and replicas R; for one-job tilings the ligand's 4-atom
+chunk count instead of L's octave), so a tiling is a function of the job
+alone.
 
   shared launches (batches, serving): for each (L, P) bucket of the screening
       sizes (L 12-80, P 1,500-9,000), gd_autotune_shared on a batch of 16
       complexes of a screen (one target, ligands of the bucket, 10 x 11 x 4
       replicas), 32 MC steps per candidate: protein tile points x persistent
       waves per CU.  The waves per CU most buckets pick is kept for all.
-  one job per launch: for each (L, P, R) bucket of the configurations' plain
-      jobs (R = 25, 220, 440, 880, 1,760), gd_autotune on two synthetic
-      complexes of the bucket's L range on the library's default streams, 128
+  one job per launch: for each (chunks, P, R) bucket of the configurations'
+      plain jobs (L 12-80 in 4-atom chunks, R = 25, 220, 440, 880, 1,760),
+      gd_autotune on a synthetic complex on the library's default streams, 128
       MC steps per candidate: atoms per lane x tile count, ranked by MC step
-      period; differing winners are settled on both complexes.
+      period.
 
 The sweep candidates are the C functions' own (geauxdock_amd/csrc/gd_batch.hip).
 """
@@ -35,8 +36,8 @@ import geauxdock_amd as g  # noqa: E402
 from geauxdock_amd import abi  # noqa: E402
 
 L_REP = {3: 14, 4: 24, 5: 46, 6: 72}          # representative L of each log2 bucket (12-80)
-# one-job tilings are keyed on half octaves of L (hlg2): two ligand sizes per bucket
-L_REPS = {7: (12, 15), 8: (16, 22), 9: (25, 31), 10: (34, 44), 11: (50, 60), 12: (66, 78)}
+# one-job tilings are keyed on the ligand's 4-atom chunk count, ceil(L / 4)
+L_REPS = {c: (4 * c,) for c in range(3, 21)}     # L = 12 .. 80
 P_REP = {10: 1700, 11: 3000, 12: 6000, 13: 9000}  # representative P of each log2 bucket (1,500-9,000)
 R_SHAPES = [(5, 5, 1), (5, 11, 4), (10, 11, 4), (20, 11, 4), (20, 11, 8)]   # 25 .. 1,760 replicas
 
@@ -48,11 +49,9 @@ def lg2(x):
     return b
 
 
-def hlg2(x):
-    """Half-octave index: 2 floor(log2 x), plus 1 from 1.5 2^k on (the
-    library's key for one-job tilings)."""
-    k = lg2(max(1, x))
-    return 2 * k + (1 if k > 0 and 2 * x >= (3 << k) else 0)
+def chunk_key(L):
+    """The library's key of one-job tilings: 4-atom chunks of the ligand."""
+    return (max(1, L) + 3) // 4
 
 
 def tune_shared(ene, steps=32, n_jobs=16):
@@ -87,13 +86,14 @@ def _period(dev, lc, pt, steps):
 
 
 def tune_plain(ene, steps=128):
-    """Each (L, P, R) bucket on two ligand sizes of its L range (same P and R):
-    gd_autotune on each; where the winners differ, every winner is timed on
-    both jobs and the one with the smallest summed period relative to each
+    """Each (chunks, P, R) bucket on its representative ligand sizes (one per
+    chunk count: L = 4 chunks) at the bucket's P and R: gd_autotune on each;
+    where several are given and their winners differ, every winner is timed
+    on every job and the one with the smallest summed period relative to each
     job's own best is kept."""
     entries = []
     for hL, Ls in L_REPS.items():
-        assert all(hlg2(L) == hL for L in Ls), Ls
+        assert all(chunk_key(L) == hL for L in Ls), Ls
         for lgP, P in P_REP.items():
             for nl, np_, nt in R_SHAPES:
                 devs, wins = [], []
@@ -127,10 +127,10 @@ HEADER = """// gd_tiling_table.h -- tiling table of the pair kernel, written by
 // `python -m geauxdock_amd.tune` from sweeps on an MI355X (gd_autotune and
 // gd_autotune_shared).  A job's tiling is looked up here by size bucket
 // (floor(log2) of its ligand atoms L, protein points P and replicas R; for
-// one-job tilings half octaves of L: hlgL = 2 floor(log2 L), plus 1 from
-// 1.5 2^floor(log2 L) on), so it is a function of the job alone, never of what
-// else the process ran.
-//   GD_TUNE_PLAIN  (hlgL, lgP, lgR) -> atoms per lane, protein tile count
+// one-job tilings the ligand's 4-atom chunk count ceil(L / 4) in place of
+// L's), so it is a function of the job alone, never of what else the process
+// ran.
+//   GD_TUNE_PLAIN  (chunks, lgP, lgR) -> atoms per lane, protein tile count
 //   GD_TUNE_SHARED (lgL, lgP)      -> protein tile points (4 atoms per lane)
 //   GD_TUNE_WAVES  ()              -> persistent pair kernel waves per CU
 // Buckets without an entry use the built-in rule (gd_runtime.hip).

@@ -1,10 +1,10 @@
 """The tiling table's lookup keys (CPU; gd_tiling_lookup touches no device).
 
-One-job tilings are keyed on half octaves of the ligand atom count,
-hlg(L) = 2 floor(log2 L) + [L >= 1.5 2^floor(log2 L)], and on octaves of P and
-R; shared-launch tilings on octaves of L and P.  Every entry of the compiled
-table must be what the library returns for a job inside its bucket, and jobs
-either side of a bucket edge must get their own bucket's entry."""
+One-job tilings are keyed on the ligand's 4-atom chunk count ceil(L / 4) and on
+octaves of P and R; shared-launch tilings on octaves of L and P.  Every entry
+of the compiled table must be what the library returns for a job inside its
+bucket, and jobs either side of a bucket edge must get their own bucket's
+entry."""
 import ctypes as C
 import math
 import os
@@ -27,12 +27,9 @@ def first_with(key_fn, key, lo=1, hi=4096):
     return next(x for x in range(lo, hi) if key_fn(x) == key)
 
 
-def test_half_octave_key():
-    for x in range(1, 5000):
-        k = tune.lg2(x)
-        assert tune.hlg2(x) == 2 * k + (1 if k > 0 and x >= 1.5 * 2 ** k else 0)
-        if x >= 2:    # between floor(2 log2 x) - 1 and floor(2 log2 x)
-            assert math.floor(2 * math.log2(x)) - 1 <= tune.hlg2(x) <= math.floor(2 * math.log2(x))
+def test_chunk_key():
+    for L in range(1, 500):
+        assert tune.chunk_key(L) == math.ceil(L / 4)
 
 
 @pytest.mark.parametrize("mode", ["GD_TUNE_PLAIN", "GD_TUNE_SHARED"])
@@ -43,7 +40,7 @@ def test_every_entry_is_what_the_library_returns(mode):
         P = 2 ** lgP
         if mode == "GD_TUNE_PLAIN":
             R = 2 ** lgR
-            for L in (first_with(tune.hlg2, kl), max(x for x in range(1, 4096) if tune.hlg2(x) == kl)):
+            for L in (first_with(tune.chunk_key, kl), max(x for x in range(1, 4096) if tune.chunk_key(x) == kl)):
                 got = lookup(L, P, R, False)
                 assert got[:2] == (lc, -(-P // min(value, P))), (kl, lgP, lgR, L)
         else:
@@ -53,9 +50,9 @@ def test_every_entry_is_what_the_library_returns(mode):
 
 def test_bucket_edges_pick_their_own_entries():
     plain = {(e[1], e[2], e[3]): e for e in tune.read_table(TABLE, ("GD_TUNE_PLAIN",))}
-    for L0, L1 in [(47, 48), (63, 64), (31, 32), (23, 24)]:
+    for L0, L1 in [(40, 41), (44, 45), (12, 13), (79, 80)]:
         for P, R in [(3000, 1760), (6000, 440)]:
-            a, b = (plain.get((tune.hlg2(L), tune.lg2(P), tune.lg2(R))) for L in (L0, L1))
+            a, b = (plain.get((tune.chunk_key(L), tune.lg2(P), tune.lg2(R))) for L in (L0, L1))
             if a is None or b is None:
                 continue
             assert lookup(L0, P, R, False)[:2] == (a[4], -(-P // min(a[5], P)))
