@@ -148,11 +148,11 @@ typedef struct {
 
 typedef struct {
   double ms_total;          /* wall time of the MC loop (host clock) */
-  double ms_pair_kernel;    /* summed HIP-event time of the timed pair-energy launches (on two
-                               streams: of both half-launches of each timed step) */
+  double ms_pair_kernel;    /* summed HIP-event time of the timed pair-energy launches (on n
+                               streams: of the n part launches of each timed step) */
   double ms_finish_kernel;  /* reserved, 0: the tail kernels are not event-timed */
-  int n_pair_launch;        /* pair-kernel launches timed (steps 0, 16, 32, ...; two per step on
-                               two streams) */
+  int n_pair_launch;        /* pair-kernel launches timed (steps 0, 16, 32, ...; n per step on
+                               n streams) */
   double pair_evals;        /* ligand-atom x protein-point evaluations done */
   double replica_steps;     /* replica x MC steps done (initial pass excluded) */
   double ms_step_period;    /* steady-state MC step period: from the first to the last timed
