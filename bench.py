@@ -198,7 +198,7 @@ def timed_plain(host, mc, reference_shape):
     t0 = time.perf_counter()
     d = g.DeviceJob(host)
     if reference_shape:
-        d.dock_stream(mc, lambda *a: None, blocking=True, stats=False)
+        d.dock_stream(mc, None, blocking=True, stats=False)
     else:
         d.dock(mc, stats=False)
     d.close()
@@ -781,7 +781,7 @@ def main():
     mc_ref = g.make_mc(steps, steps_per_dump=200, steps_per_exchange=STEPS_PER_EXCHANGE,
                        move_mode="gaussian", seed=SEED, axes=AXES)
     t0 = time.perf_counter()
-    dev.dock_stream(mc_ref, lambda *a: None, blocking=True, stats=False)
+    dev.dock_stream(mc_ref, None, blocking=True, stats=False)
     wall_ref = time.perf_counter() - t0
     line["reference_shape"] = {
         "value": job.n_rep * steps / wall_ref, "unit": "replica-steps/s",

@@ -244,8 +244,12 @@ class DeviceJob:
     def dock_stream(self, mc, on_dump, blocking=False, stats=True):
         """gd_dock_stream: on_dump(dump, step_begin, rec, count) receives each
         dump's accept records as numpy copies, rec[n_rep][cap][17] and
-        count[n_rep] (cap = steps_per_dump)."""
+        count[n_rep] (cap = steps_per_dump).  on_dump=None takes the records to
+        the library's host buffer only (the reference's per-dump copy), with no
+        further copy."""
         def fn(_user, dump, step_begin, n_rep, cap, rec, count):
+            if on_dump is None:
+                return
             r = np.ctypeslib.as_array(rec, (n_rep * cap * abi.REC_W,)).reshape(n_rep, cap, abi.REC_W)
             c = np.ctypeslib.as_array(count, (n_rep,))
             on_dump(dump, step_begin, r.copy(), c.copy())
