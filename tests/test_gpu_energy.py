@@ -90,8 +90,9 @@ def test_golden_line(golden_job, golden_dev, oracle):
 
 
 def test_golden_random_poses(golden_job, golden_dev, oracle):
-    conf, pose = random_poses(golden_job, max(512, N_POSES), seed=11)
-    _parity(golden_job, golden_dev, oracle, conf, pose, "1a07C1 x512")
+    n = max(512, N_POSES)
+    conf, pose = random_poses(golden_job, n, seed=11)
+    _parity(golden_job, golden_dev, oracle, conf, pose, "1a07C1 x%d" % n)
 
 
 def test_golden_all_conformers_native(golden_job, golden_dev, oracle):
