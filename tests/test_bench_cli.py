@@ -72,3 +72,66 @@ def test_batched_workload_shapes():
     assert all(d["P"] == 4000 and 12 <= d["L"] <= 80 and d["nlig"] * d["nprt"] * d["ntmp"] == 440 for d in z)
     z = bench._sizes(np.random.default_rng(0), 64, "c4")
     assert all(1500 <= d["P"] <= 9000 and d["nlig"] * d["nprt"] * d["ntmp"] == 220 for d in z)
+
+
+def _write_counter_csv(path, counter, rows):
+    hdr = ("Correlation_Id,Dispatch_Id,Agent_Id,Queue_Id,Process_Id,Thread_Id,Grid_Size,Kernel_Id,Kernel_Name,"
+           "Workgroup_Size,LDS_Block_Size,Scratch_Size,VGPR_Count,Accum_VGPR_Count,SGPR_Count,Counter_Name,"
+           "Counter_Value,Start_Timestamp,End_Timestamp\n")
+    with open(path, "w") as f:
+        f.write(hdr)
+        for i, (name, grid, val) in enumerate(rows):
+            f.write('%d,%d,"Agent 1",1,1,1,%d,1,"%s",512,0,0,64,0,80,"%s",%f,0,1\n'
+                    % (i + 1, i + 1, grid, name, counter, val))
+
+
+def test_traffic_summary_per_step(tmp_path):
+    """HBM bytes per MC step: two equal half-grid dispatches summed, each counter
+    scaled by its reading for a 256 MiB copy (fetch reads half: x2)."""
+    sys.path.insert(0, ROOT)
+    import json
+    import bench
+    pair, copy = "void gd_pair_kernel<4>(GdJob const*, GdItem const*, int)", "gd_copy_kernel(float4)"
+    dirs = {k: tmp_path / k for k in ("f", "w", "cf", "cw")}
+    for d in dirs.values():
+        d.mkdir()
+    _write_counter_csv(dirs["f"] / "c2_counter_collection.csv", "FETCH_SIZE",
+                       [(pair, 1000, 100.0), (pair, 1000, 100.0), (pair, 2000, 150.0)])
+    _write_counter_csv(dirs["w"] / "c2_counter_collection.csv", "WRITE_SIZE",
+                       [(pair, 1000, 50.0), (pair, 1000, 50.0), (pair, 2000, 100.0)])
+    _write_counter_csv(dirs["cf"] / "copy_counter_collection.csv", "FETCH_SIZE", [(copy, 64, 131072.0)])
+    _write_counter_csv(dirs["cw"] / "copy_counter_collection.csv", "WRITE_SIZE", [(copy, 64, 262144.0)])
+    out = tmp_path / "t.json"
+    r = bench.traffic_summary(str(dirs["f"]), str(dirs["w"]), str(dirs["cf"]), str(dirs["cw"]), str(out))
+    assert abs(r["fetch_scale"] - 2.0) < 1e-12 and abs(r["write_scale"] - 1.0) < 1e-12
+    half = (100.0 * 2.0 + 50.0) * 1024.0
+    assert abs(r["hbm_bytes_per_launch"] - 2 * half) < 1e-6
+    assert abs(r["hbm_bytes_whole_launch"] - (150.0 * 2.0 + 100.0) * 1024.0) < 1e-6
+    assert json.load(open(out))["kernel"] == pair
+
+
+def test_trace_summary_two_streams(tmp_path):
+    """Step period from the first stream's half-launch starts; span of a step's
+    two halves on different streams; whole-grid dispatches apart."""
+    sys.path.insert(0, ROOT)
+    import bench
+    pair = "void gd_pair_kernel<4>(GdJob const*, GdItem const*, int)"
+    rows = []
+    cid = 0
+    for k in range(5):                       # 5 two-stream steps, period 100, halves 60 long, offset 30
+        t = 1000 + 100 * k
+        for sid, t0 in ((1, t), (2, t + 30)):
+            cid += 1
+            rows.append((cid, sid, 500, t0, t0 + 60))
+    cid += 1
+    rows.append((cid, 1, 1000, 2000, 2090))  # a whole-grid swap pass
+    path = tmp_path / "kt.csv"
+    with open(path, "w") as f:
+        f.write("Kernel_Name,Correlation_Id,Stream_Id,Grid_Size_X,Start_Timestamp,End_Timestamp\n")
+        for c, sid, grid, a, b in rows:
+            f.write('"%s",%d,%d,%d,%d,%d\n' % (pair, c, sid, grid, a * 1000, b * 1000))
+    r = bench.trace_summary(str(path))["pair"]
+    assert r["two_stream_steps"] == 5 and r["whole_job_dispatches"] == 1
+    assert abs(r["two_stream_step_period_us"] - 100.0) < 1e-9
+    assert abs(r["two_stream_step_span_mean_us"] - 90.0) < 1e-9
+    assert abs(r["half_dispatch_mean_us"] - 60.0) < 1e-9 and abs(r["whole_job_mean_us"] - 90.0) < 1e-9
