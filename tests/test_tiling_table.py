@@ -57,3 +57,17 @@ def test_bucket_edges_pick_their_own_entries():
                 continue
             assert lookup(L0, P, R, False)[:2] == (a[4], -(-P // min(a[5], P)))
             assert lookup(L1, P, R, False)[:2] == (b[4], -(-P // min(b[5], P)))
+
+
+def test_table_round_trip(tmp_path):
+    """tune.py writes a header the reader (and the C table) take back unchanged."""
+    entries = tune.read_table(TABLE, ("GD_TUNE_PLAIN", "GD_TUNE_SHARED", "GD_TUNE_WAVES"))
+    out = tmp_path / "t.h"
+    tune.write_table(entries, str(out), "round trip")
+    assert tune.read_table(str(out), ("GD_TUNE_PLAIN", "GD_TUNE_SHARED", "GD_TUNE_WAVES")) == entries
+    text = open(out).read()
+    assert "kTunedTable[]" in text and "// round trip" in text
+    # one entry per key of each mode
+    for mode in ("GD_TUNE_PLAIN", "GD_TUNE_SHARED"):
+        keys = [e[1:4] for e in entries if e[0] == mode]
+        assert len(keys) == len(set(keys)), mode
