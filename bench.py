@@ -657,7 +657,7 @@ def main():
                                                                       "pair_kernel_c2_traffic.json"))))
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+        print("bench.py: measures one GPU; --gpus %d is not supported "
               "(jobs shard across GPUs without communication)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
