@@ -172,7 +172,9 @@ def write_table(entries, path, device):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "csrc", "gd_tiling_table.h"))
-    ap.add_argument("--shared-only", action="store_true")
+    ap.add_argument("--shared-only", action="store_true",
+                    help="re-tune the shared-launch entries, keeping the one-job entries of the table the "
+                         "library was built with")
     ap.add_argument("--plain-only", action="store_true",
                     help="re-tune the one-job entries, keeping the shared-launch entries of the "
                          "table the library was built with")
@@ -182,13 +184,19 @@ def main():
     ncu, clk, hbm = C.c_int(), C.c_int(), C.c_size_t()
     g.lib().gd_device_info(name, 128, C.byref(ncu), C.byref(clk), C.byref(hbm))
     ene = g.load_enepara()
+    if args.plain_only and args.shared_only:
+        ap.error("--plain-only and --shared-only exclude each other")
+    src = os.path.join(HERE, "csrc", "gd_tiling_table.h")   # the table the library was built with
     if args.plain_only:
-        src = os.path.join(HERE, "csrc", "gd_tiling_table.h")   # the table the library was built with
         entries = read_table(src, ("GD_TUNE_SHARED", "GD_TUNE_WAVES"))
         print("kept %d shared-launch entries of %s" % (len(entries), src), flush=True)
     else:
         entries = tune_shared(ene)
-    if not args.shared_only:
+    if args.shared_only:
+        kept = read_table(src, ("GD_TUNE_PLAIN",))
+        print("kept %d one-job entries of %s" % (len(kept), src), flush=True)
+        entries += kept
+    else:
         entries += tune_plain(ene)
     write_table(entries, args.out, "tuned on %s" % name.value.decode())
     print("wrote %s (%d entries)" % (args.out, len(entries)), flush=True)
