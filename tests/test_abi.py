@@ -130,3 +130,41 @@ def test_dropin_layout_matches_reference():
     bad = {k: (mine.get(k), v) for k, v in ref.items() if mine.get(k) != v}
     assert not bad, bad
     assert len(ref) > 100
+
+
+def test_host_side_argument_errors_and_csv_sink(tmp_path):
+    """Entry points that touch no device: argument checks return GD_ERR_ARG,
+    an unwritable record file GD_ERR_IO with a message, and the PrintRecord sink
+    writes replica rep_lo..rep_hi (inclusive) of a dump's records, count[r]
+    entries each, as "%3d %d" + " %.4f" x 10."""
+    t = abi.gd_tiling()
+    assert g.lib().gd_tiling_lookup(0, 3000, 1760, 0, C.byref(t)) == abi.GD_ERR_ARG
+    assert g.lib().gd_tiling_lookup(40, 3000, 1760, 0, None) == abi.GD_ERR_ARG
+    assert g.lib().gd_tiling_lookup(40, 3000, 1760, 1, C.byref(t)) == abi.GD_OK
+    assert t.atoms_per_lane == 4 and 1 <= t.tile_points <= 3000 and t.waves_per_cu > 0
+    assert g.lib().gd_job_set_streams(None, 2) == abi.GD_ERR_ARG
+    w = C.c_void_p()
+    assert g.lib().gd_record_csv_open(str(tmp_path / "x.txt").encode(), 3, 1, 1, C.byref(w)) == abi.GD_ERR_ARG
+    assert g.lib().gd_record_csv_open(str(tmp_path / "no" / "x.txt").encode(), 0, 1, 1,
+                                      C.byref(w)) == abi.GD_ERR_IO
+    assert b"cannot open" in g.lib().gd_last_error()
+    path = tmp_path / "rec.txt"
+    assert g.lib().gd_record_csv_open(str(path).encode(), 1, 2, 1, C.byref(w)) == abi.GD_OK
+    n_rep, cap = 4, 3
+    rec = np.zeros((n_rep, cap, abi.REC_W), np.float32)
+    for r in range(n_rep):
+        for k in range(cap):
+            rec[r, k, 0] = 10 * k + r
+            rec[r, k, 7:17] = np.arange(10) * 0.5 + r + 0.01 * k
+    count = np.array([3, 2, 0, 1], np.int32)
+    g.lib().gd_record_csv_write(w, 0, 0, n_rep, cap, rec.ctypes.data_as(C.POINTER(C.c_float)),
+                                count.ctypes.data_as(C.POINTER(C.c_int)))
+    assert g.lib().gd_record_csv_close(w) == abi.GD_OK
+    lines = path.read_text().splitlines()
+    assert lines[0].split() == ["rep", "step", "vdw", "ele", "pmf", "psp", "hdb", "hpc", "kde", "lhm", "dst",
+                                "total"]
+    want = []
+    for r in (1, 2):                      # rep_lo..rep_hi, with count[r] entries each
+        for k in range(count[r]):
+            want.append("%3d %d" % (r, int(rec[r, k, 0])) + "".join(" %.4f" % v for v in rec[r, k, 7:17]))
+    assert lines[1:] == want
