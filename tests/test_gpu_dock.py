@@ -52,6 +52,21 @@ def test_short_trajectory_matches_oracle(small_job, oracle, mode):
     assert e < 1e-3, e
 
 
+@pytest.mark.parametrize("P,L", [(8, 2), (37, 3)])
+def test_small_ragged_trajectory(ene, oracle, P, L):
+    """A ligand of 2 or 3 atoms (one padded chunk) on a protein of 8 or 37
+    points, with exchange on every axis, follows the oracle's chains."""
+    job = g.synth_complex(P, L, 3, 2, 3, floor_temp=0.05, ceiling_temp=0.3, seed=P, ene=ene)
+    dev = g.DeviceJob(job)
+    mc = g.make_mc(30, steps_per_exchange=5, move_mode="gaussian", seed=P + 1, axes=("temp", "lig", "prt"))
+    out, _ = dev.dock(mc)
+    o_out, _, _ = oracle.dock(job, mc, threads=8)
+    same = (out.n_accept == o_out.n_accept) & (np.abs(out.movematrix - o_out.movematrix).max(1) < 1e-3)
+    assert same.mean() >= 0.9, same.mean()
+    e = np.abs(out.energy[same] - o_out.energy[same]).max()
+    assert e < 1e-3, e
+
+
 def test_acceptance_statistics_match_oracle(ene, oracle):
     job = g.synth_complex(400, 16, 4, 4, 4, floor_temp=0.05, ceiling_temp=0.5, seed=3, ene=ene)
     dev = g.DeviceJob(job)

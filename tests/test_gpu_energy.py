@@ -110,6 +110,35 @@ def test_synthetic_parity(ene, oracle, P, L, nl, np_, seed):
     _parity(job, dev, oracle, conf, pose, "synthetic P=%d L=%d" % (P, L))
 
 
+@pytest.mark.parametrize("P,L", [(8, 2), (37, 3), (64, 5), (130, 7)])
+def test_small_ragged_parity(ene, oracle, P, L):
+    """Ragged shapes: ligand chunks padded with empty atoms (L not a multiple of
+    the chunk width), proteins smaller than one work group's eight tiles, and
+    one-point tiles, which leave whole tiles without points, KDE anchors or
+    MCS rows (the tile count is rounded up to a multiple of eight)."""
+    job = g.synth_complex(P, L, 3, 2, 2, seed=P + L, ene=ene)
+    dev = g.DeviceJob(job)
+    conf, pose = random_poses(job, 96, seed=P, trans=1.0)
+    first = None
+    for tiling in [None, (4, 1), (8, 3), (16, 5)]:
+        if tiling:
+            dev.set_tiling(*tiling)
+            lc, tp = dev.tiling()
+            assert (lc, tp) == (tiling[0], min(tiling[1], P)), (lc, tp)
+        _parity(job, dev, oracle, conf, pose, "P=%d L=%d tiling %s" % (P, L, tiling),
+                check_transform=tiling is None)
+        terms, _, _ = dev.eval(conf, pose)
+        if first is None:
+            first = terms
+        else:   # only the summation order changes with the tiling
+            assert np.abs(terms - first).max() < 1e-4
+
+
+def test_eval_of_no_poses(golden_dev):
+    terms, raw, _ = golden_dev.eval(np.zeros((0, 2), np.int32), np.zeros((0, 6), np.float32))
+    assert terms.shape == (0, 10) and raw.shape == (0, 9)
+
+
 def test_far_away_and_edges(golden_job, golden_dev, oracle):
     # far from the protein: pair terms vanish, dst dominates; exact zero pose
     conf = np.zeros((3, 2), np.int32)
