@@ -181,13 +181,26 @@ static float stream_float(const std::string &s) {
   return v;
 }
 
+// A loader's failure: the message goes to gd_last_error() and to stderr.
+static gd_status loader_error(gd_status st, const std::string &msg) {
+  gdr::set_error(msg);
+  std::fprintf(stderr, "geauxdock_amd: %s\n", msg.c_str());
+  return st;
+}
+
 // ---------------------------------------------------------------------------
 // force field
 extern "C" gd_status gd_load_enepara(const char *paras, const char *weight, const char *nor_a,
                                      const char *nor_b, gd_enepara *out) {
   if (!paras || !weight || !nor_a || !nor_b || !out) return GD_ERR_ARG;
   std::vector<std::string> lines;
-  if (!read_lines(paras, lines)) return GD_ERR_IO;
+  if (!read_lines(paras, lines))
+    return loader_error(GD_ERR_IO, std::string("cannot open parameter file ") + paras);
+  int lineno = 0;
+  const auto bad = [&](gd_status st, const char *what) {
+    return loader_error(st, std::string("parameter file ") + paras + " line " + std::to_string(lineno) +
+                                ": " + what);
+  };
   // raw tables (EnePara0 of geauxdock.h:279-295), zero-initialised, owned by
   // this call (concurrent loads share nothing)
   struct RawTables {
@@ -200,38 +213,39 @@ extern "C" gd_status gd_load_enepara(const char *paras, const char *weight, cons
   float lj[3] = {0, 0, 0}, el[2] = {0, 0}, kde = 0;
   float w[GD_NTERM] = {0}, a_para[GD_NTERM] = {0}, b_para[GD_NTERM] = {0};
   for (const std::string &ln : lines) {
+    ++lineno;
     if (ln.size() <= 3) continue;
     const std::string key = ln.substr(0, 3);
     const std::vector<std::string> t = tokens(ln);
     if (key == "VDW" || key == "PMF" || key == "HDB") {
-      if (t.size() < 5) return GD_ERR_IO;
+      if (t.size() < 5) return bad(GD_ERR_IO, "record needs a point type, a ligand type and two values");
       const int p = pnt_code(t[1]), l = lig_code(t[2]);
-      if (p < 0 || l >= GD_NLT) return GD_ERR_TYPE;
+      if (p < 0 || l >= GD_NLT) return bad(GD_ERR_TYPE, "unknown point or ligand type");
       float(*tab)[GD_NLT][2] = key == "VDW" ? vdw : (key == "PMF" ? pmf : hdb);
       tab[p][l][0] = fatof(t[3]);
       tab[p][l][1] = fatof(t[4]);
     } else if (key == "PLJ") {
       for (int i = 0; i < 3 && i + 1 < (int)t.size(); ++i) lj[i] = fatof(t[i + 1]);
     } else if (key == "ELE") {
-      if (t.size() < 3) return GD_ERR_IO;
+      if (t.size() < 3) return bad(GD_ERR_IO, "ELE record needs a point and a value");
       int idx;
       if (t[1].size() >= 2 && t[1][1] == 'C')
         idx = res_code_one(t[1].substr(0, 1)) + 30;  // residue CA entry (load.C:1245-1249)
       else
         idx = pnt_code(t[1]);
-      if (idx < 0 || idx >= GD_NELE) return GD_ERR_TYPE;
+      if (idx < 0 || idx >= GD_NELE) return bad(GD_ERR_TYPE, "unknown ELE point");
       ele[idx] = fatof(t[2]);
     } else if (key == "PEL") {
       for (int i = 0; i < 2 && i + 1 < (int)t.size(); ++i) el[i] = fatof(t[i + 1]);
     } else if (key == "HPP") {
-      if (t.size() < 3) return GD_ERR_IO;
+      if (t.size() < 3) return bad(GD_ERR_IO, "HPP record needs a residue and a value");
       const int r = res_code_one(t[1]);
-      if (r < 0) return GD_ERR_TYPE;
+      if (r < 0) return bad(GD_ERR_TYPE, "unknown HPP residue");
       hpp[r] = fatof(t[2]);
     } else if (key == "HPL") {
-      if (t.size() < 4) return GD_ERR_IO;
+      if (t.size() < 4) return bad(GD_ERR_IO, "HPL record needs a ligand type and two values");
       const int l = lig_code(t[1]);
-      if (l >= GD_NLT) return GD_ERR_TYPE;
+      if (l >= GD_NLT) return bad(GD_ERR_TYPE, "unknown HPL ligand type");
       hpl[l][0] = fatof(t[2]);
       hpl[l][1] = fatof(t[3]);
     } else if (key == "KDE") {
@@ -250,8 +264,11 @@ extern "C" gd_status gd_load_enepara(const char *paras, const char *weight, cons
     hpp[i] = std::min(1.0f, std::max(-1.0f, v));
   }
   std::vector<std::string> wl, al, bl;
-  if (!read_lines(weight, wl) || !read_lines(nor_a, al) || !read_lines(nor_b, bl))
-    return GD_ERR_IO;
+  const char *const files[3] = {weight, nor_a, nor_b};
+  std::vector<std::string> *const dst[3] = {&wl, &al, &bl};
+  for (int k = 0; k < 3; ++k)
+    if (!read_lines(files[k], *dst[k]))
+      return loader_error(GD_ERR_IO, std::string("cannot open parameter file ") + files[k]);
   for (size_t i = 0; i < wl.size() && i < GD_NTERM; ++i) w[i] = stream_float(wl[i]);
   for (size_t i = 0; i < al.size() && i < GD_NTERM; ++i) a_para[i] = stream_float(al[i]);
   for (size_t i = 0; i < bl.size() && i < GD_NTERM; ++i) b_para[i] = stream_float(bl[i]);
@@ -407,21 +424,27 @@ static gd_status load_lhm(const char *path, const std::string &lig_id, int natom
     err = std::string("cannot open restraint file ") + path;
     return GD_ERR_IO;
   }
+  int lineno = 0;
+  const auto bad = [&](const char *what) {
+    err = std::string("restraint file ") + path + " line " + std::to_string(lineno) + ": " + what;
+    return GD_ERR_IO;
+  };
   for (const std::string &s : ln) {
+    ++lineno;
     if (s.size() <= 3) continue;
     const std::string key = s.substr(0, 3);
     if (key == "KDE") {
       const auto t = tokens(s);
-      if (t.size() < 5) return GD_ERR_IO;
+      if (t.size() < 5) return bad("KDE record needs a type and three coordinates");
       h.kt.push_back(lig_code(t[1]));
       h.kx.push_back(fatof(t[2]));
       h.ky.push_back(fatof(t[3]));
       h.kz.push_back(fatof(t[4]));
     } else if (key == "PSP") {
       const auto t = tokens(s);
-      if (t.size() < 4) return GD_ERR_IO;
+      if (t.size() < 4) return bad("PSP record needs a residue, a type and a value");
       const int res = std::atoi(t[1].c_str()), lt = lig_code(t[2]);
-      if (res < 0) return GD_ERR_IO;
+      if (res < 0) return bad("PSP residue number is negative");
       if (lt < GD_NLT) h.psp[{res, lt}] = fatof(t[3]);
     } else if (key == "MCS") {
       const auto t = tokens(s);
@@ -433,12 +456,9 @@ static gd_status load_lhm(const char *path, const std::string &lig_id, int natom
         tcc = fatof(t[2]);
         ncol = std::atoi(t[3].c_str());
         for (int ia = 0; ia < ncol; ++ia) {
-          if ((size_t)(ia * 4 + 7) >= t.size()) return GD_ERR_IO;
+          if ((size_t)(ia * 4 + 7) >= t.size()) return bad("MCS record has fewer columns than it declares");
           const int loc = std::atoi(t[ia * 4 + 4].c_str()) - 1;
-          if (loc < 0 || loc >= 63) {
-            err = "MCS position out of bounds";
-            return GD_ERR_IO;
-          }
+          if (loc < 0 || loc >= 63) return bad("MCS position out of bounds");
           if (loc >= natom) continue;
           x[loc] = fatof(t[ia * 4 + 5]);
           y[loc] = fatof(t[ia * 4 + 6]);
@@ -678,23 +698,20 @@ extern "C" gd_status gd_load_complex(const char *sdf, const char *ff, const char
   Lig0 lig;
   gd_status st = load_ligand(sdf, lig, err);
   if (st != GD_OK) {
-    std::fprintf(stderr, "geauxdock_amd: %s\n", err.c_str());
     delete h;
-    return st;
+    return loader_error(st, err);
   }
   Lhm0 lhm;
   st = load_lhm(ff, ligand_id, lig.natom, lhm, err);
   if (st != GD_OK) {
-    std::fprintf(stderr, "geauxdock_amd: %s\n", err.c_str());
     delete h;
-    return st;
+    return loader_error(st, err);
   }
   Prt0 prt;
   st = load_protein(pdb, prt, err);
   if (st != GD_OK) {
-    std::fprintf(stderr, "geauxdock_amd: %s\n", err.c_str());
     delete h;
-    return st;
+    return loader_error(st, err);
   }
   h->ene = *ene;
   std::vector<std::vector<float>> confs;
@@ -885,6 +902,19 @@ static gd_status synth(int P, int L, int n_lig, int n_prt, int n_tmp, float floo
   // ---- protein: residue chains by random walk, expanded into points
   Prt0 pr;
   const double pocket_r = pocket_r_fixed > 0.0 ? pocket_r_fixed : lig_radius + 2.5;
+  // Points go in the box outside the pocket sphere.  A box whose corners lie
+  // within the sphere has no room at all; one whose room is a sliver would
+  // take unbounded rejection sampling, so chain starts get a fixed budget.
+  const auto no_room = [&]() {
+    char msg[160];
+    std::snprintf(msg, sizeof msg,
+                  "synthetic protein of %d points (box %.1f A) has no room outside a pocket of "
+                  "radius %.1f A",
+                  P, box, pocket_r);
+    delete h;
+    return loader_error(GD_ERR_ARG, msg);
+  };
+  if (pocket_r >= 0.5 * std::sqrt(3.0) * box) return no_room();
   std::vector<V3> cas;  // CA of each residue
   {
   Rng &rng = rp;
@@ -898,9 +928,11 @@ static gd_status synth(int P, int L, int n_lig, int n_prt, int n_tmp, float floo
   std::vector<V3> pts;
   while ((int)pr.t.size() < P) {
     if (new_chain) {
+      int tries = 0;
       do {
         ca = V3{rng.u() * box, rng.u() * box, rng.u() * box};
-      } while (!inside(ca));
+      } while (!inside(ca) && ++tries < (1 << 20));
+      if (!inside(ca)) return no_room();
       new_chain = false;
     } else {
       V3 nx;

@@ -6,6 +6,11 @@
 
 #include "../../include/geauxdock_amd.h"
 
+namespace gdr {
+// the text gd_last_error() returns on this thread (defined with the runtime)
+void set_error(const std::string &s);
+}  // namespace gdr
+
 struct gd_host_job {
   int n_lig = 0, n_prt = 0, n_tmp = 0;
   int L = 0, P = 0, K = 0, M = 0;

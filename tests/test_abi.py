@@ -168,3 +168,31 @@ def test_host_side_argument_errors_and_csv_sink(tmp_path):
         for k in range(count[r]):
             want.append("%3d %d" % (r, int(rec[r, k, 0])) + "".join(" %.4f" % v for v in rec[r, k, 7:17]))
     assert lines[1:] == want
+
+
+@pytest.mark.timeout(120)
+def test_loader_and_generator_errors(ene, tmp_path):
+    """Inputs the loaders and generators cannot use fail at once with a status
+    and a message in gd_last_error(): a synthetic protein whose box has no
+    room outside the pocket (the screen's fixed 14 A pocket, or a ligand wider
+    than the box), a restraint record with missing columns (the message names
+    its line), and a missing parameter file.  The smallest screen protein that
+    fits still builds."""
+    from tests.conftest import GOLDEN_DATA as d
+    for make in (lambda: g.synth_screen(8, 1, 2, 6, 2, 2, 2, ene=ene),
+                 lambda: g.synth_screen(80, 3, 20, 4, 2, 2, 2, ene=ene),
+                 lambda: g.synth_complex(8, 80, 2, 2, 2, ene=ene)):
+        with pytest.raises(g.GDError, match="no room outside a pocket"):
+            make()
+    job = g.synth_screen(100, 1, 12, 2, 2, 2, 2, ene=ene)
+    assert job.desc.prt_npoint == 100
+    ff = open(os.path.join(d, "1a07C1-0.8.ff")).read().splitlines()
+    k = next(i for i, s in enumerate(ff) if s.startswith("KDE"))
+    ff[k] = "KDE C.3 1.0"
+    bad = tmp_path / "bad.ff"
+    bad.write_text("\n".join(ff) + "\n")
+    with pytest.raises(g.GDError, match="line %d: KDE record" % (k + 1)):
+        g.load_complex(os.path.join(d, "1a07C1.sdf"), str(bad), os.path.join(d, "1a07C-1.pdb"), "1a07C1",
+                       ene=ene)
+    with pytest.raises(g.GDError, match="cannot open parameter file"):
+        g.load_enepara(str(tmp_path))
