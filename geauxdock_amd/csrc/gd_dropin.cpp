@@ -39,6 +39,29 @@ void die(gd_status st, const char *what) {
 
 DropinCtx *ctx_of(Complex **c) { return reinterpret_cast<DropinCtx *>(c[0]); }
 
+// The sizes must fit the Complex's fixed capacities (size.h), and n_rep must be
+// the ensemble product, or the reads of the Complex and the writes of the
+// replicas and records would run past their arrays: fail as the reference's
+// error paths do, with a message.
+void check_sizes(const ComplexSize &sz, const char *where) {
+  const struct {
+    const char *name;
+    int v, cap;
+  } f[] = {{"n_lig", sz.n_lig, GD_REF_MAX_CONF_LIG},   {"n_prt", sz.n_prt, GD_REF_MAX_CONF_PRT},
+           {"n_tmp", sz.n_tmp, GD_REF_MAX_TMP},         {"n_rep", sz.n_rep, GD_REF_MAX_REP},
+           {"lig_natom", sz.lig_natom, GD_REF_MAXLIG},  {"prt_npoint", sz.prt_npoint, GD_REF_MAXPRO},
+           {"kde_npoint", sz.kde_npoint, GD_REF_MAXKDE}, {"mcs_nrow", sz.mcs_nrow, GD_REF_MAX_MCS_ROW}};
+  for (const auto &e : f)
+    if (e.v < 0 || e.v > e.cap) {
+      std::fprintf(stderr, "%s failed: ComplexSize.%s = %d is outside 0..%d\n", where, e.name, e.v, e.cap);
+      std::exit(EXIT_FAILURE);
+    }
+  if ((long long)sz.n_lig * sz.n_prt * sz.n_tmp != sz.n_rep) {
+    std::fprintf(stderr, "%s failed: ComplexSize.n_rep = %d is not n_lig x n_prt x n_tmp\n", where, sz.n_rep);
+    std::exit(EXIT_FAILURE);
+  }
+}
+
 // Reference Complex (fixed capacities, geauxdock.h:391-439) -> gd_job_desc.
 void to_desc(Complex *ch, DropinCtx *x, gd_job_desc *d) {
   const ComplexSize &sz = ch->size;
@@ -172,6 +195,7 @@ void DeviceFree(Complex **c, Record **r, gd_rng_state **s) {
 
 // One device: the whole replica range (prepair.C:92-143 with NGPU = 1).
 void SetParaT(Complex *ch, ParaT **pt) {
+  check_sizes(ch->size, "SetParaT");
   const int n_rep = ch->size.n_rep;
   pt[0]->rep_begin = 0;
   pt[0]->rep_end = n_rep - 1;
@@ -187,6 +211,7 @@ void CopyH2D(Complex *ch, Complex **cd, ParaT **pt) {
     gd_job_destroy(x->job);
     x->job = nullptr;
   }
+  check_sizes(ch->size, "CopyH2D");
   gd_job_desc d;
   to_desc(ch, x, &d);
   const gd_status st = gd_job_create(&d, &x->job);

@@ -196,3 +196,42 @@ def test_loader_and_generator_errors(ene, tmp_path):
                        ene=ene)
     with pytest.raises(g.GDError, match="cannot open parameter file"):
         g.load_enepara(str(tmp_path))
+
+
+_SETPARAT = r'''
+import ctypes as C, json, struct, sys
+sys.path.insert(0, sys.argv[1])
+import geauxdock_amd as g
+lay = json.load(open(sys.argv[2]))
+f = getattr(g.lib(), "_Z8SetParaTP7ComplexPP5ParaT")
+f.restype = None
+cx = C.create_string_buffer(lay["Complex.sizeof"])
+sizes = [int(v) for v in sys.argv[3].split(",")]   # n_lig n_prt n_tmp n_rep L P K M
+C.memmove(C.addressof(cx) + lay["Complex.size"], struct.pack("8i", *sizes), 32)
+pt = C.create_string_buffer(lay["ParaT.sizeof"])
+f(cx, (C.c_void_p * 1)(C.addressof(pt)))
+print("rep_end", struct.unpack_from("i", pt.raw, lay["ParaT.rep_end"])[0])
+'''
+
+
+@pytest.mark.parametrize("sizes,msg", [
+    ("2,2,2,8,40,300,100,5", None),
+    ("2,2,2,8,70,300,100,5", "ComplexSize.lig_natom = 70 is outside 0..64"),
+    ("2,21,2,84,40,300,100,5", "ComplexSize.n_prt = 21 is outside 0..20"),
+    ("2,2,2,9,40,300,100,5", "ComplexSize.n_rep = 9 is not n_lig x n_prt x n_tmp"),
+    ("2,2,2,8,40,300,100,-1", "ComplexSize.mcs_nrow = -1 is outside 0..512"),
+])
+def test_dropin_sizes_beyond_capacity_fail_loudly(sizes, msg):
+    """The drop-in reads the reference Complex's fixed-capacity arrays by its
+    ComplexSize: sizes outside the capacities (or an n_rep that is not the
+    ensemble product) end the program with a message instead of reading past
+    the arrays (SetParaT touches no device)."""
+    import subprocess
+    import sys
+    layout = os.path.join(ROOT, "tests", "golden", "ref_layout.json")
+    r = subprocess.run([sys.executable, "-c", _SETPARAT, ROOT, layout, sizes], capture_output=True, text=True,
+                       timeout=120)
+    if msg is None:
+        assert r.returncode == 0 and "rep_end 7" in r.stdout, r.stderr
+    else:
+        assert r.returncode == 1 and msg in r.stderr, (r.returncode, r.stderr)
