@@ -424,13 +424,27 @@ def batched(kind, n_jobs, mc_steps, batch_size=0, plain_sample=0, roofline_steps
     return line
 
 
+def fifo_latencies(arrivals, service):
+    """Latency (arrival -> result) of each request at a server that runs one
+    job at a time in arrival order, given each job's run time alone: a job
+    starts when it has arrived and the previous job has finished."""
+    lat, free = [], 0.0
+    for a, t in zip(arrivals, service):
+        free = max(free, a) + t
+        lat.append(free - a)
+    return np.array(lat)
+
+
 def serve(rates, n_jobs, step_scale):
     """Serving: Poisson arrivals, admission by device footprint, continuous
     batching; p50/p99 latency (arrival -> result) and jobs/s per rate, and the
-    plain path's jobs/s on the same requests (both shapes)."""
+    plain path on the same requests (both shapes): jobs/s, and the p50/p99
+    latency of a one-job-at-a-time server on the same arrivals, from each
+    job's measured plain run time."""
     import geauxdock_amd as g
     from geauxdock_amd import abi
     res = []
+    arrivals = {}
     rng = np.random.default_rng(5)
     sizes = _sizes(rng, n_jobs, "c5")
     steps = [int(rng.integers(2000, 10001) * step_scale) for _ in range(n_jobs)]
@@ -439,6 +453,7 @@ def serve(rates, n_jobs, step_scale):
                      axes=AXES) for i in range(n_jobs)]
     for lam in rates:
         arr_t = np.cumsum(np.random.default_rng(int(lam)).exponential(1.0 / lam, n_jobs))
+        arrivals[lam] = arr_t
         cfg = abi.gd_server_config(hbm_budget=200 << 30, max_active=256, sweep_steps=50)
         s = C.c_void_p()
         g.check(g.lib().gd_server_create(C.byref(cfg), C.byref(s)), "gd_server_create")
@@ -473,17 +488,25 @@ def serve(rates, n_jobs, step_scale):
         res.append({"rate_per_s": lam, "jobs": n_jobs, "jobs_per_s": n_jobs / wall,
                     "p50_latency_s": float(np.percentile(lat, 50)),
                     "p99_latency_s": float(np.percentile(lat, 99))})
-    tp = tr = 0.0
+    t_plain, t_ref = [], []
     for i in range(n_jobs):
-        tp += timed_plain(hosts[i], mcs[i], False)
+        t_plain.append(timed_plain(hosts[i], mcs[i], False))
         mc_r = g.make_mc(steps[i], steps_per_dump=200, steps_per_exchange=100, move_mode="gaussian",
                          seed=7, job_id=i, axes=AXES)
-        tr += timed_plain(hosts[i], mc_r, True)
+        t_ref.append(timed_plain(hosts[i], mc_r, True))
         if (i + 1) % 50 == 0:
-            print("[c5] plain: %d jobs in %.1f s" % (i + 1, tp + tr), file=sys.stderr, flush=True)
+            print("[c5] plain: %d jobs in %.1f s" % (i + 1, sum(t_plain) + sum(t_ref)), file=sys.stderr,
+                  flush=True)
+    for r in res:
+        for key, t in (("plain", t_plain), ("plain_reference_shape", t_ref)):
+            lat = fifo_latencies(arrivals[r["rate_per_s"]], t)
+            r[key + "_p50_latency_s"] = float(np.percentile(lat, 50))
+            r[key + "_p99_latency_s"] = float(np.percentile(lat, 99))
     return {"workload": "c5-serving", "step_scale": step_scale, "per_rate": res,
-            "plain_jobs_per_s": n_jobs / tp, "plain_reference_shape_jobs_per_s": n_jobs / tr,
-            "plain_note": "same requests, one job per launch sequence, back to back"}
+            "plain_jobs_per_s": n_jobs / sum(t_plain), "plain_reference_shape_jobs_per_s": n_jobs / sum(t_ref),
+            "plain_note": "same requests, one job per launch sequence, back to back; plain latencies are "
+                          "those of a one-job-at-a-time server on the same arrival trace, from each job's "
+                          "measured run time alone"}
 
 
 def _counter_rows(run_dir, counter):

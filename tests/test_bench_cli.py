@@ -4,6 +4,8 @@ import os
 import subprocess
 import sys
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -135,3 +137,11 @@ def test_trace_summary_two_streams(tmp_path):
     assert abs(r["two_stream_step_period_us"] - 100.0) < 1e-9
     assert abs(r["two_stream_step_span_mean_us"] - 90.0) < 1e-9
     assert abs(r["half_dispatch_mean_us"] - 60.0) < 1e-9 and abs(r["whole_job_mean_us"] - 90.0) < 1e-9
+
+
+def test_fifo_latencies():
+    """The plain server's latencies: one job at a time, in arrival order."""
+    import bench
+    np.testing.assert_allclose(bench.fifo_latencies([0, 0, 0], [1, 2, 3]), [1, 3, 6])    # a queue builds
+    np.testing.assert_allclose(bench.fifo_latencies([0, 10], [1, 1]), [1, 1])            # idle in between
+    np.testing.assert_allclose(bench.fifo_latencies([0, 0.5, 5], [2, 2, 1]), [2, 3.5, 1])
