@@ -435,7 +435,7 @@ def fifo_latencies(arrivals, service):
     return np.array(lat)
 
 
-def serve(rates, n_jobs, step_scale):
+def serve(rates, n_jobs, step_scale, max_active=256):
     """Serving: Poisson arrivals, admission by device footprint, continuous
     batching; p50/p99 latency (arrival -> result) and jobs/s per rate, and the
     plain path on the same requests (both shapes): jobs/s, and the p50/p99
@@ -454,7 +454,7 @@ def serve(rates, n_jobs, step_scale):
     for lam in rates:
         arr_t = np.cumsum(np.random.default_rng(int(lam)).exponential(1.0 / lam, n_jobs))
         arrivals[lam] = arr_t
-        cfg = abi.gd_server_config(hbm_budget=200 << 30, max_active=256, sweep_steps=50)
+        cfg = abi.gd_server_config(hbm_budget=200 << 30, max_active=max_active, sweep_steps=50)
         s = C.c_void_p()
         g.check(g.lib().gd_server_create(C.byref(cfg), C.byref(s)), "gd_server_create")
         t0 = time.perf_counter()
@@ -502,7 +502,7 @@ def serve(rates, n_jobs, step_scale):
             lat = fifo_latencies(arrivals[r["rate_per_s"]], t)
             r[key + "_p50_latency_s"] = float(np.percentile(lat, 50))
             r[key + "_p99_latency_s"] = float(np.percentile(lat, 99))
-    return {"workload": "c5-serving", "step_scale": step_scale, "per_rate": res,
+    return {"workload": "c5-serving", "step_scale": step_scale, "max_active": max_active, "per_rate": res,
             "plain_jobs_per_s": n_jobs / sum(t_plain), "plain_reference_shape_jobs_per_s": n_jobs / sum(t_ref),
             "plain_note": "same requests, one job per launch sequence, back to back; plain latencies are "
                           "those of a one-job-at-a-time server on the same arrival trace, from each job's "
@@ -668,6 +668,7 @@ def main():
                     help="c3/c4: jobs per batched launch sequence (0: all jobs in one)")
     ap.add_argument("--plain-sample", type=int, default=32,
                     help="c3/c4: plain legs on this many randomly drawn jobs (0: every job)")
+    ap.add_argument("--max-active", type=int, default=256, help="c5: jobs the server keeps active at once")
     ap.add_argument("--no-extra", action="store_true",
                     help="default run: the headline line only, without its c1/c3 keys")
     args = ap.parse_args()
@@ -698,7 +699,7 @@ def main():
                                  args.plain_sample, cpu=not args.no_cpu)), flush=True)
         return
     if args.workload == "c5":
-        print(json.dumps(serve([50, 200, 800], args.jobs, args.mc_steps / 10000.0)), flush=True)
+        print(json.dumps(serve([50, 200, 800], args.jobs, args.mc_steps / 10000.0, args.max_active)), flush=True)
         return
     name = C.create_string_buffer(128)
     ncu, clk, hbm = C.c_int(), C.c_int(), C.c_size_t()
