@@ -435,7 +435,7 @@ def fifo_latencies(arrivals, service):
     return np.array(lat)
 
 
-def serve(rates, n_jobs, step_scale, max_active=256):
+def serve(rates, n_jobs, step_scale, max_active=32):
     """Serving: Poisson arrivals, admission by device footprint, continuous
     batching; p50/p99 latency (arrival -> result) and jobs/s per rate, and the
     plain path on the same requests (both shapes): jobs/s, and the p50/p99
@@ -668,7 +668,7 @@ def main():
                     help="c3/c4: jobs per batched launch sequence (0: all jobs in one)")
     ap.add_argument("--plain-sample", type=int, default=32,
                     help="c3/c4: plain legs on this many randomly drawn jobs (0: every job)")
-    ap.add_argument("--max-active", type=int, default=256, help="c5: jobs the server keeps active at once")
+    ap.add_argument("--max-active", type=int, default=32, help="c5: jobs the server keeps active at once")
     ap.add_argument("--no-extra", action="store_true",
                     help="default run: the headline line only, without its c1/c3 keys")
     args = ap.parse_args()
