@@ -415,7 +415,7 @@ gd_status apply_tiling(gd_job *j, int lc, int ptile) {
   // cheapest items; within a chunk, tile groups, then replica blocks, so items
   // dispatched together read the same force-field rows and KDE anchors.  The
   // order changes no result: every item writes its own partial-sum slot.
-  // (Measured on a 3,000-point, 1,760-replica job: 118.3 us against 120.8 us for
+  // (Measured on a 3,000-point, 1,760-replica job: 117.3 us against 119.8 us for
   // replica-block-major order, and 121.0 us for the reverse chunk order.)
   std::vector<int> chunk(g.nL), wkde(g.nL, 0);
   for (int a = 0; a < g.nL; ++a) {
